@@ -1,0 +1,179 @@
+/*
+ * tsr.h - C-ABI of the MI355X-native TwoScaleRich time-stepping library (libtsr.so).
+ *
+ * Three groups of entry points:
+ *   1. Reference signatures, so a caller written for the reference links unchanged.
+ *      Each cites the reference declaration it replaces.
+ *   2. Problem set-up: the BASELINE configurations and the reference's own input files,
+ *      producing the batch view (tsr_types.h) that the device path consumes.
+ *   3. A device-resident solver: state stays in HBM between steps.
+ *
+ * Errors: functions returning int give 0 on success; nonzero codes are listed per
+ * function.  tsr_last_error() returns a message for the most recent failure.
+ */
+#ifndef TSR_H
+#define TSR_H
+
+#include "tsr_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ================================ 1. reference signatures ============================== */
+
+/* RHS on the GPU.  Replaces Gfunc_parallel / Gfunc_serial (include/main.h:21-22; selected
+ * at main.cpp:198-213).  u and g are host arrays of user_data.N doubles.  The problem is
+ * the batch view registered for (user_data.macro_mesh.elements, N) by the set-up calls,
+ * or, if none is registered, the reference's semantics (one micro mesh, soil B in every
+ * micro element, cells hosted by macro elements when N = M_nodes + M_elements*m_nodes,
+ * by macro nodes when N = M_nodes*(1 + m_nodes)).  Returns 0, 1 if g is not finite,
+ * 2 on a device or set-up error. */
+int Gfunc_hip(double* g, double* u, user_data_struct user_data);
+
+/* One accepted EXPREM step, C twin of the reference's C++ exprem (include/main.h:29-32):
+ * t and stats by pointer.  When Gfunc == Gfunc_hip the whole step runs on the device;
+ * any other Gfunc is called on the host for every RHS evaluation while the Krylov
+ * arithmetic stays on the device.  Returns 0 or -1 (main.cpp:319-322). */
+int exprem_c(tsr_gfunc_t Gfunc, int N, double* t, double tend, double* tspan, double* tspan_solns, double* u0,
+             user_data_struct user_data, exprem_options_struct options, exprem_stats_struct* stats);
+
+/* phi_1(alpha*H[0:m,0:m]) by the (6,6) Pade approximant with scaling and squaring.
+ * src/phipade.cpp:15 (H leading dimension MaxKrylov+1, phiH MaxKrylov).  Returns 0, or 1
+ * when the Pade denominator is singular (the reference exits instead, phipade.cpp:162-167). */
+int phipade(double* phiH, double* H, double alpha, int m, int MaxKrylov);
+
+/* sqrt(mean((PhiError_i / (AbsTol + RelTol |u_i|))^2)) over host arrays.  include/exprem.h:27 */
+double weighted_norm(double* PhiError, double* u, double AbsTol, double RelTol, int N);
+
+/* include/exprem.h:29: reports (stderr) a step below MinStep. */
+void check_step(double dt, double MinStep);
+
+/* include/exprem.h:30: step size to retry with after the RHS returned flag (halved if
+ * flag != 0, unchanged otherwise). */
+double check_flag_Gfunc(int flag, double dt);
+
+/* Defaults of src/set_exprem_options.cpp:16-26 (C twin of set_exprem_options). */
+int set_exprem_options_c(exprem_options_struct* options);
+
+/* Initialise stats before the first step (the reference leaves them unset, main.cpp:257;
+ * exprem_c also does this itself when *t == 0). */
+void exprem_stats_init(exprem_stats_struct* stats, const exprem_options_struct* options, double* u0);
+
+/* ===================================== 2. set-up ======================================= */
+
+typedef struct tsr_problem tsr_problem;
+
+/* Workloads: 0 plain CPU-runnable (macro 21^2, micro 11^2, t in [0,0.1] d), 1 plain GPU
+ * scale (101^2 / 21^2, [0,1] d), 2 ragged cells (101^2; micro 11/16/21/31 per cell, 1-3
+ * random inclusions), 3 batch of independent scenarios (64 x (41^2 / 16^2), sampled
+ * soils), 4 large (201^2 / 31^2, one random inclusion per cell).  macro_n, micro_r and
+ * n_scen override the sizes when > 0 (micro_r is ignored for workload 2).  seed = 0 uses
+ * the workload's fixed seed. */
+enum { TSR_CFG_PLAIN_SMALL = 0, TSR_CFG_PLAIN = 1, TSR_CFG_RAGGED = 2, TSR_CFG_SCENARIOS = 3, TSR_CFG_LARGE = 4 };
+tsr_problem* tsr_problem_config(int cfg, int macro_n, int micro_r, int n_scen, unsigned long long seed);
+
+/* The reference's input files (tests/Richards_Equation/: *.msh, Keff.txt).  host_kind
+ * TSR_HOST_ELEMENT reproduces main.cpp:260.  Returns NULL on a read error. */
+tsr_problem* tsr_problem_reference(const char* macro_msh, const char* micro_msh, const char* keff_txt,
+                                   const soil_struct* soilA, const soil_struct* soilB, double boundary_flux,
+                                   const int* macro_bc5, int model_type, double h0, int host_kind);
+
+void                    tsr_problem_free(tsr_problem* p);
+const tsr_batch_view*   tsr_problem_view(const tsr_problem* p);
+const user_data_struct* tsr_problem_user_data(const tsr_problem* p);   /* N = whole batch */
+long long               tsr_problem_N(const tsr_problem* p);
+void                    tsr_problem_initial(const tsr_problem* p, double* u0);
+double                  tsr_problem_tend(const tsr_problem* p);
+void                    tsr_problem_options(const tsr_problem* p, exprem_options_struct* o);
+/* Geometry-distinct bytes a G call must read beyond u and g (per-cell inclusion
+ * parameters, per-scenario soils and Keff tables); the algorithmic traffic of one G
+ * call is 16*N plus this. */
+long long               tsr_problem_distinct_bytes(const tsr_problem* p);
+
+const char* tsr_last_error(void);
+
+/* ============================== 3. device-resident solver ============================== */
+
+typedef struct tsr_solver tsr_solver;
+
+/* Per-step record (the reference's statistics.txt line, main.cpp:324-329, extended with
+ * what the step controller saw). */
+typedef struct tsr_step_record {
+    double t;            /* time after the step */
+    double tau;          /* step taken */
+    double tau_proposed; /* step proposed before error control */
+    double err;          /* WRMS error estimate of the accepted attempt */
+    double beta;         /* ||G(u)||_2 at the step start */
+    double hnorm;        /* ||H_m||_inf (Hessenberg of the accepted step) */
+    double resid;        /* last Arnoldi residual h_{m+1,m} */
+    double head_mean, head_min, head_max, sat_mean, sat_min, sat_max;  /* macro head / saturation stats */
+    int    m;            /* Krylov dimension of phi_1 */
+    int    mD;           /* Krylov dimension of the error estimate */
+    int    rejects;      /* rejected attempts before acceptance */
+    int    fevals;       /* RHS evaluations in this step */
+    int    controller;   /* 0 classical, 1 learned proposal accepted, 2 learned proposal fell back */
+    int    m_cap;        /* Krylov cap used */
+} tsr_step_record;
+
+typedef struct tsr_run_stats {
+    double t;
+    int    steps, rejects, fevals, failed;
+    double wall_s;
+    double gfunc_ms, arnoldi_ms;   /* device time in the RHS and in the Krylov kernels */
+} tsr_run_stats;
+
+/* Creates device copies of the problem (mesh templates, soils, Keff tables) and the
+ * Krylov workspace for opt->MaxKrylov (plus learned-controller cap up to 60).  Returns
+ * NULL on failure. */
+tsr_solver* tsr_solver_create(const tsr_batch_view* view, const exprem_options_struct* opt);
+void        tsr_solver_free(tsr_solver* s);
+int         tsr_solver_set_state(tsr_solver* s, const double* u_host, double t);
+int         tsr_solver_get_state(tsr_solver* s, double* u_host, double* t);
+void        tsr_solver_stats(const tsr_solver* s, exprem_stats_struct* st);
+/* Reset statistics and step-size proposal (tau = options.InitStep). */
+void        tsr_solver_reset(tsr_solver* s);
+
+/* g = G(u) with u, g host arrays (copies through HBM). */
+int tsr_solver_gfunc_host(tsr_solver* s, double* g, const double* u);
+/* g_dev = G(u_dev) on device pointers; repeat launches `reps` times (timing). */
+int tsr_solver_gfunc_dev(tsr_solver* s, double* g_dev, const double* u_dev, int reps);
+/* Device time (ms) of the last tsr_solver_gfunc_dev call per launch, from HIP events on
+ * the solver's stream. */
+double tsr_solver_last_gfunc_ms(const tsr_solver* s);
+
+/* One accepted step on the device state (the exprem contract).  rec may be NULL. */
+int tsr_solver_step(tsr_solver* s, double tend, tsr_step_record* rec);
+/* Steps until t >= tend or max_steps; records (if non-NULL) receives up to max_records. */
+int tsr_solver_run(tsr_solver* s, double tend, int max_steps, tsr_run_stats* out, tsr_step_record* records,
+                   int max_records);
+
+/* Step controller: 0 classical (EXPREM's law), 1 learned (MLP proposal, classical fallback).
+ * tsr_solver_set_mlp loads FP32 master weights (layout of tsr_mlp_param_count). */
+int tsr_solver_set_controller(tsr_solver* s, int mode);
+int tsr_solver_set_mlp(tsr_solver* s, const float* params, const float* feat_mean, const float* feat_std);
+
+/* Raw device pointers (for tests and the bench): u, a scratch vector of length N. */
+double* tsr_solver_dev_u(tsr_solver* s);
+double* tsr_solver_dev_scratch(tsr_solver* s, int which);
+long long tsr_solver_N(const tsr_solver* s);
+int tsr_device_sync(void);
+/* Device copy helpers for host harnesses. */
+int tsr_h2d(double* dst_dev, const double* src_host, long long n);
+int tsr_d2h(double* dst_host, const double* src_dev, long long n);
+
+#ifdef __cplusplus
+}
+
+/* C++-linkage reference signatures (include/main.h:23-32), for callers compiled
+ * against the reference's main.h. */
+int macro_mesh_properties(const char* mesh_file, macro_mesh_struct& macro_mesh, int* macro_bc);
+int micro_mesh_properties(const char* mesh_file, micro_mesh_struct& micro_mesh);
+int read_effective_conductivity(const char* Keff_file, Keff_struct& effective_conductivity);
+int set_exprem_options(exprem_options_struct& options);
+int exprem(int (*Gfunc)(double* g, double* u, user_data_struct user_data), int N, double& t, double tend,
+           double* tspan, double* tspan_solns, double* u0, user_data_struct user_data, exprem_options_struct options,
+           exprem_stats_struct& stats);
+#endif
+
+#endif /* TSR_H */

@@ -1,0 +1,60 @@
+"""The C-ABI library loads on a machine without a GPU and exports every function that
+include/tsr/tsr.h declares (no compute calls)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import twoscalerich_amd as T
+from twoscalerich_amd import _ffi as F
+
+HDR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "tsr", "tsr.h")
+
+
+def _declared_c_functions():
+    src = open(HDR).read()
+    c_part = src.split("#ifdef __cplusplus\n}")[0]
+    c_part = re.sub(r"/\*.*?\*/", "", c_part, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\([^;{]*\)\s*;", c_part)
+    return sorted({n for n in names if n not in ("typedef",) and not n.startswith("(")})
+
+
+def test_every_declared_c_function_is_exported():
+    lib = T.lib()
+    names = _declared_c_functions()
+    assert len(names) > 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_cpp_linkage_reference_signatures_are_exported():
+    out = subprocess.run(["nm", "-D", "-C", "--defined-only", F.LIB_PATH], capture_output=True, text=True).stdout
+    for sig in ("macro_mesh_properties(char const*, macro_mesh_struct&, int*)",
+                "micro_mesh_properties(char const*, micro_mesh_struct&)",
+                "read_effective_conductivity(char const*, Keff_struct&)",
+                "set_exprem_options(exprem_options_struct&)",
+                "exprem(int (*)(double*, double*, user_data_struct), int, double&, double, double*, double*, double*, "
+                "user_data_struct, exprem_options_struct, exprem_stats_struct&)"):
+        assert sig in out, sig
+
+
+def test_ctypes_declarations_cover_the_library():
+    assert not F.declare(T.lib())
+
+
+def test_struct_sizes_match_c_layout():
+    # the reference's structs on x86-64 (LP64)
+    assert C.sizeof(F.macro_mesh_struct) == 3 * 4 + 4 + 13 * 8
+    assert C.sizeof(F.soil_struct) == 48
+    assert C.sizeof(F.exprem_options_struct) == 8 * 2 + 4 * 2 + 8 * 2 + 8 + 8 * 4
+    assert C.sizeof(F.exprem_stats_struct) == 8 + 8 + 8 + 8 + 4 + 4
+
+
+def test_host_dense_routines_and_helpers():
+    L = T.lib()
+    assert L.check_flag_Gfunc(0, 0.5) == 0.5 and L.check_flag_Gfunc(3, 0.5) == 0.25
+    o = F.exprem_options_struct()
+    L.set_exprem_options_c(C.byref(o))
+    assert (o.AbsTol, o.RelTol, o.MinKrylov, o.MaxKrylov, o.MinStep, o.MaxStep, o.MaxNumSteps, o.SafetyFac,
+            o.MinStepDecFac, o.MaxStepIncFac, o.InitStep) == (1e-5, 1e-5, 2, 100, 1e-10, 1000.0, 100000, 0.8, 0.1,
+                                                            1.1, 1e-3)

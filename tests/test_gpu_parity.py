@@ -1,0 +1,124 @@
+"""GPU parity: the HIP right-hand side and EXPREM against the CPU oracle.
+
+G: ||g_gpu - g_cpu||_inf / ||g_cpu||_inf <= 1e-12 (normwise, because g vanishes at
+equilibrium).  EXPREM: the same steps from the same state agree to 1e-6 relative.
+"""
+import numpy as np
+import pytest
+
+import twoscalerich_amd as T
+from twoscalerich_amd import _ffi as F
+from oracle.oracle import Oracle
+
+from helpers import rel_inf, seeded_state, reference_problem
+
+pytestmark = pytest.mark.gpu
+
+G_TOL = 1e-12
+
+CASES = [
+    # (name, cfg, macro_n, micro_r, n_scen)
+    ("plain_small", F.CFG_PLAIN_SMALL, 0, 0, 0),
+    ("plain_21_21", F.CFG_PLAIN, 21, 21, 0),
+    ("ragged_11", F.CFG_RAGGED, 11, 0, 0),
+    ("scenarios_4", F.CFG_SCENARIOS, 9, 8, 4),
+    ("large_disc_31", F.CFG_LARGE, 7, 31, 0),
+]
+
+
+@pytest.mark.parametrize("name,cfg,mn,mr,ns", CASES, ids=[c[0] for c in CASES])
+def test_gfunc_matches_oracle(name, cfg, mn, mr, ns):
+    p = T.Problem.config(cfg, mn, mr, ns)
+    s = T.Solver(p)
+    o = Oracle(p.view)
+    for seed in (1, 2):
+        u = seeded_state(p, seed)
+        g_gpu, rc = s.gfunc(u)
+        g_ref, rc_ref = o.gfunc(u)
+        assert rc == 0 and rc_ref == 0
+        assert np.all(np.isfinite(g_gpu))
+        err = rel_inf(g_gpu, g_ref)
+        print(f"{name} seed {seed}: N={p.N} rel_inf={err:.3e} |g|max={np.abs(g_ref).max():.3e}", flush=True)
+        assert err <= G_TOL
+        # discrete water balance of the GPU g: sum(storage * g) == boundary inflow
+        _, inflow, sg = o.water_balance(u, g_gpu)
+        assert abs(sg - inflow) <= 1e-11 * max(abs(inflow), abs(sg), 1e-300)
+    s.close()
+    o.close()
+
+
+@pytest.mark.parametrize("model", [1, 2, 3])
+def test_gfunc_reference_meshes(model):
+    """The reference's own test meshes: quad macro mesh, unstructured inclusion micro
+    mesh, cells hosted by macro elements, BC types 1/3/2 (main.cpp:180-184)."""
+    p = reference_problem(model=model)
+    s = T.Solver(p)
+    o = Oracle(p.view)
+    u = seeded_state(p, 7, -3.0, -0.2)
+    g_gpu, rc = s.gfunc(u)
+    g_ref, _ = o.gfunc(u)
+    err = rel_inf(g_gpu, g_ref)
+    print(f"reference meshes model {model}: N={p.N} rel_inf={err:.3e}", flush=True)
+    assert rc == 0 and err <= G_TOL
+    s.close()
+    o.close()
+
+
+def test_gfunc_hip_reference_signature():
+    """Gfunc_hip(g, u, user_data) with the reference's by-value struct."""
+    p = T.Problem.config(F.CFG_PLAIN_SMALL)
+    u = seeded_state(p, 3)
+    g = np.empty_like(u)
+    L = T.lib()
+    rc = L.Gfunc_hip(T._dp(g), T._dp(u), p.ud.contents)
+    o = Oracle(p.view)
+    g_ref, _ = o.gfunc(u)
+    assert rc == 0 and rel_inf(g, g_ref) <= G_TOL
+    o.close()
+
+
+def test_exprem_steps_match_oracle():
+    """20 EXPREM steps of the plain small workload on the GPU and on the CPU oracle."""
+    p = T.Problem.config(F.CFG_PLAIN_SMALL)
+    opt = p.options()
+    s = T.Solver(p, opt)
+    u0 = p.initial()
+    s.set_state(u0, 0.0)
+    o = Oracle(p.view, p.ud)
+    u_cpu = u0.copy()
+    st = F.exprem_stats_struct()
+    t_cpu = 0.0
+    tend = p.tend()
+    for k in range(20):
+        rc, recs = s.step(tend)
+        assert rc == 0
+        rc2, t_cpu, last = o.exprem_step(p.ud.contents, u_cpu, t_cpu, tend, opt, st)
+        assert rc2 == 0
+        r = recs[0]
+        print(f"step {k}: gpu tau={r.tau:.6e} m={r.m} mD={r.mD} rej={r.rejects} err={r.err:.4f} | "
+              f"cpu tau={last[4]:.6e} m={int(last[0])} mD={int(last[1])} err={last[3]:.4f}", flush=True)
+    u_gpu, t_gpu = s.get_state()
+    assert abs(t_gpu - t_cpu) <= 1e-9 * max(t_cpu, 1e-30)
+    err = rel_inf(u_gpu, u_cpu)
+    print(f"after 20 steps t={t_gpu:.6e}: rel_inf(u)={err:.3e}", flush=True)
+    assert err <= 1e-6
+    s.close()
+    o.close()
+
+
+def test_exprem_c_reference_signature():
+    """exprem_c drop-in: two steps with Gfunc_hip on host arrays, stats filled."""
+    p = T.Problem.config(F.CFG_PLAIN_SMALL)
+    L = T.lib()
+    opt = p.options()
+    u = p.initial()
+    t = T.C.c_double(0.0)
+    st = F.exprem_stats_struct()
+    tspan = np.array([1.0, 0.05])
+    sol = np.zeros(2 * p.N)
+    gf = T.C.cast(L.Gfunc_hip, T.C.c_void_p)
+    for _ in range(2):
+        rc = L.exprem_c(gf, p.N, T.C.byref(t), p.tend(), T._dp(tspan), T._dp(sol), T._dp(u), p.ud.contents, opt,
+                        T.C.byref(st))
+        assert rc == 0
+    assert st.NumSteps == 2 and st.FuncEvals > 2 and 0 < t.value < 0.05

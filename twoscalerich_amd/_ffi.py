@@ -1,0 +1,151 @@
+"""ctypes mirror of include/tsr/tsr_types.h and the C-ABI of libtsr.so.
+
+The struct layouts are the reference's plain-C carriers (include/macro_mesh.h,
+micro_mesh.h, Keff.h, soil.h, user_data.h, exprem_options.h, exprem_stats.h) plus the
+batch view; field order and types must match the C header exactly.
+"""
+import ctypes as C
+import os
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class macro_mesh_struct(C.Structure):
+    _fields_ = [("no_verts", C.c_int), ("no_nodes", C.c_int), ("no_elements", C.c_int),
+                ("elements", _ip), ("nodes", _dp), ("shape_funcs", _dp), ("normals", _dp),
+                ("boundary_edge_midpts", _dp), ("boundary_edge_normals", _dp), ("boundary_edge_lens", _dp),
+                ("boundary_condition_edges", _ip), ("CV_area", _dp), ("SCV_area", _dp), ("midx", _dp),
+                ("midy", _dp), ("centroid_elements", _dp)]
+
+
+class micro_mesh_struct(C.Structure):
+    _fields_ = [("no_nodes", C.c_int), ("no_elements", C.c_int), ("elements", _ip), ("boundary_nodes", _ip),
+                ("nodes", _dp), ("shape_funcs", _dp), ("normals", _dp), ("boundary_edges", _ip),
+                ("boundary_edge_midpts", _dp), ("boundary_edge_normals", _dp), ("boundary_edge_lens", _dp),
+                ("centroid_elements", _dp), ("CV_area", _dp), ("midx", _dp), ("midy", _dp),
+                ("cell_area", C.c_double), ("vectors", _dp), ("no_verts", C.c_int), ("element_area", _dp)]
+
+
+class Keff_struct(C.Structure):
+    _fields_ = [("epsilonA", C.c_double), ("no_hvalues", C.c_int), ("logr", C.c_double),
+                ("logh_first", C.c_double), ("logh_values", _dp), ("Keff_values", _dp)]
+
+
+class soil_struct(C.Structure):
+    _fields_ = [("Ksat", C.c_double), ("thetar", C.c_double), ("thetas", C.c_double),
+                ("alpha", C.c_double), ("n", C.c_double), ("m", C.c_double)]
+
+
+class user_data_struct(C.Structure):
+    _fields_ = [("macro_mesh", macro_mesh_struct), ("micro_mesh", micro_mesh_struct),
+                ("effective_conductivity", Keff_struct), ("soilA", soil_struct), ("soilB", soil_struct),
+                ("boundary_flux", C.c_double), ("N", C.c_int), ("model_type", C.c_int),
+                ("NUM_THREADS", C.c_uint)]
+
+
+class exprem_options_struct(C.Structure):
+    _fields_ = [("AbsTol", C.c_double), ("RelTol", C.c_double), ("MinKrylov", C.c_int), ("MaxKrylov", C.c_int),
+                ("MinStep", C.c_double), ("MaxStep", C.c_double), ("MaxNumSteps", C.c_int),
+                ("SafetyFac", C.c_double), ("MinStepDecFac", C.c_double), ("MaxStepIncFac", C.c_double),
+                ("InitStep", C.c_double)]
+
+
+class exprem_stats_struct(C.Structure):
+    _fields_ = [("CurrentStep", C.c_double), ("CurrentTime", C.c_double), ("CurrentKryDim", C.c_int),
+                ("CurrentSoln", _dp), ("NumSteps", C.c_int), ("FuncEvals", C.c_int)]
+
+
+class tsr_scenario(C.Structure):
+    _fields_ = [("ud", user_data_struct), ("host_kind", C.c_int), ("n_cells", C.c_int),
+                ("first_cell", C.c_int), ("pad_", C.c_int), ("u_offset", C.c_longlong),
+                ("dirichlet_head", C.c_double)]
+
+
+class tsr_batch_view(C.Structure):
+    _fields_ = [("n_scen", C.c_int), ("n_tmpl", C.c_int), ("n_cells", C.c_int), ("max_discs", C.c_int),
+                ("scen", C.POINTER(tsr_scenario)), ("tmpl", C.POINTER(micro_mesh_struct)),
+                ("tmpl_soil", C.POINTER(C.POINTER(C.c_ubyte))), ("cell_tmpl", _ip),
+                ("cell_offset", C.POINTER(C.c_longlong)), ("cell_discs", _dp), ("Ss", C.c_double),
+                ("N", C.c_longlong)]
+
+
+class tsr_step_record(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("t", "tau", "tau_proposed", "err", "beta", "hnorm", "resid",
+                                          "head_mean", "head_min", "head_max", "sat_mean", "sat_min", "sat_max")] + \
+               [(n, C.c_int) for n in ("m", "mD", "rejects", "fevals", "controller", "m_cap")]
+
+
+class tsr_run_stats(C.Structure):
+    _fields_ = [("t", C.c_double), ("steps", C.c_int), ("rejects", C.c_int), ("fevals", C.c_int),
+                ("failed", C.c_int), ("wall_s", C.c_double), ("gfunc_ms", C.c_double), ("arnoldi_ms", C.c_double)]
+
+
+GFUNC_T = C.CFUNCTYPE(C.c_int, _dp, _dp, user_data_struct)
+
+HOST_ELEMENT, HOST_NODE = 0, 1
+CFG_PLAIN_SMALL, CFG_PLAIN, CFG_RAGGED, CFG_SCENARIOS, CFG_LARGE = 0, 1, 2, 3, 4
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libtsr.so")
+
+
+def declare(lib):
+    """Argument and result types of every exported C function of libtsr.so."""
+    P = C.c_void_p
+    sig = {
+        "Gfunc_hip": (C.c_int, [_dp, _dp, user_data_struct]),
+        "exprem_c": (C.c_int, [C.c_void_p, C.c_int, _dp, C.c_double, _dp, _dp, _dp, user_data_struct,
+                               exprem_options_struct, C.POINTER(exprem_stats_struct)]),
+        "phipade": (C.c_int, [_dp, _dp, C.c_double, C.c_int, C.c_int]),
+        "weighted_norm": (C.c_double, [_dp, _dp, C.c_double, C.c_double, C.c_int]),
+        "check_step": (None, [C.c_double, C.c_double]),
+        "check_flag_Gfunc": (C.c_double, [C.c_int, C.c_double]),
+        "set_exprem_options_c": (C.c_int, [C.POINTER(exprem_options_struct)]),
+        "exprem_stats_init": (None, [C.POINTER(exprem_stats_struct), C.POINTER(exprem_options_struct), _dp]),
+        "tsr_problem_config": (P, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong]),
+        "tsr_problem_reference": (P, [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(soil_struct),
+                                      C.POINTER(soil_struct), C.c_double, _ip, C.c_int, C.c_double, C.c_int]),
+        "tsr_problem_free": (None, [P]),
+        "tsr_problem_view": (C.POINTER(tsr_batch_view), [P]),
+        "tsr_problem_user_data": (C.POINTER(user_data_struct), [P]),
+        "tsr_problem_N": (C.c_longlong, [P]),
+        "tsr_problem_initial": (None, [P, _dp]),
+        "tsr_problem_tend": (C.c_double, [P]),
+        "tsr_problem_options": (None, [P, C.POINTER(exprem_options_struct)]),
+        "tsr_problem_distinct_bytes": (C.c_longlong, [P]),
+        "tsr_last_error": (C.c_char_p, []),
+        "tsr_solver_create": (P, [C.POINTER(tsr_batch_view), C.POINTER(exprem_options_struct)]),
+        "tsr_solver_free": (None, [P]),
+        "tsr_solver_set_state": (C.c_int, [P, _dp, C.c_double]),
+        "tsr_solver_get_state": (C.c_int, [P, _dp, _dp]),
+        "tsr_solver_stats": (None, [P, C.POINTER(exprem_stats_struct)]),
+        "tsr_solver_reset": (None, [P]),
+        "tsr_solver_gfunc_host": (C.c_int, [P, _dp, _dp]),
+        "tsr_solver_gfunc_dev": (C.c_int, [P, C.c_void_p, C.c_void_p, C.c_int]),
+        "tsr_solver_last_gfunc_ms": (C.c_double, [P]),
+        "tsr_solver_step": (C.c_int, [P, C.c_double, C.POINTER(tsr_step_record)]),
+        "tsr_solver_run": (C.c_int, [P, C.c_double, C.c_int, C.POINTER(tsr_run_stats),
+                                     C.POINTER(tsr_step_record), C.c_int]),
+        "tsr_solver_set_controller": (C.c_int, [P, C.c_int]),
+        "tsr_solver_set_mlp": (C.c_int, [P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "tsr_solver_dev_u": (C.c_void_p, [P]),
+        "tsr_solver_dev_scratch": (C.c_void_p, [P, C.c_int]),
+        "tsr_solver_N": (C.c_longlong, [P]),
+        "tsr_device_sync": (C.c_int, []),
+        "tsr_h2d": (C.c_int, [C.c_void_p, _dp, C.c_longlong]),
+        "tsr_d2h": (C.c_int, [_dp, C.c_void_p, C.c_longlong]),
+    }
+    missing = []
+    for name, (res, args) in sig.items():
+        try:
+            f = getattr(lib, name)
+        except AttributeError:
+            missing.append(name)
+            continue
+        f.restype = res
+        f.argtypes = args
+    return missing
+
+
+EXPORTED = None  # filled lazily: names declare() knows about

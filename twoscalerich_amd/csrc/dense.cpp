@@ -1,0 +1,179 @@
+// dense.cpp - small FP64 dense routines on the m x m Hessenberg matrix of the Krylov
+// process (m <= 64): phi_1 by the reference's (6,6) Pade scheme, exp by (6,6) Pade with
+// scaling and squaring, and phi_k(tau H) e1 from the exponential of an augmented matrix.
+//
+// phipade follows src/phipade.cpp:15-217: scale A = alpha*H by 2^-s with
+// s = max(ceil(1 + log2 ||A||_inf), 0), (6,6) Pade approximant of phi_1 with the
+// coefficients of Hochbruck, Lubich & Selhofer (SISC 19(5) 1998, p.1567), LU solve, then
+// s doublings phi(2A) = (e^A + I) phi(A) / 2, e^{2A} = (e^A)^2.  The coefficient table
+// must equal the published one.  Differences from the reference: a singular
+// denominator returns 1 (the reference calls exit), and m = 1 with H = 0 returns 1
+// (the reference's (e^a - 1) a / log(e^a) / a is 0/0 there).
+#include "tsr_internal.h"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace tsr {
+
+namespace {
+
+// C <- alpha * A * B + beta * C, n x n column-major.
+void mm(int n, double alpha, const double* A, const double* B, double beta, double* C, double* tmp)
+{
+    for (int j = 0; j < n; ++j) {
+        double* t = tmp + (size_t)j * n;
+        for (int i = 0; i < n; ++i) t[i] = 0.0;
+        for (int k = 0; k < n; ++k) {
+            const double b = B[k + (size_t)j * n];
+            if (b == 0.0) continue;
+            const double* a = A + (size_t)k * n;
+            for (int i = 0; i < n; ++i) t[i] += a[i] * b;
+        }
+    }
+    const size_t nn = (size_t)n * n;
+    if (beta == 0.0) for (size_t i = 0; i < nn; ++i) C[i] = alpha * tmp[i];
+    else for (size_t i = 0; i < nn; ++i) C[i] = alpha * tmp[i] + beta * C[i];
+}
+
+double inf_norm(int n, const double* A)
+{
+    double r = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int j = 0; j < n; ++j) s += std::fabs(A[i + (size_t)j * n]);
+        if (!(s <= r)) r = s;
+    }
+    return r;
+}
+
+// Gaussian elimination with partial pivoting; X overwrites B (n x nrhs).
+bool solve(int n, double* A, double* B, int nrhs)
+{
+    std::vector<int> p(n);
+    for (int k = 0; k < n; ++k) {
+        int piv = k;
+        double big = std::fabs(A[k + (size_t)k * n]);
+        for (int i = k + 1; i < n; ++i) {
+            const double v = std::fabs(A[i + (size_t)k * n]);
+            if (v > big) { big = v; piv = i; }
+        }
+        if (!(big > 0.0)) return false;
+        p[k] = piv;
+        if (piv != k)
+            for (int j = 0; j < n; ++j) std::swap(A[k + (size_t)j * n], A[piv + (size_t)j * n]);
+        const double inv = 1.0 / A[k + (size_t)k * n];
+        for (int i = k + 1; i < n; ++i) A[i + (size_t)k * n] *= inv;
+        for (int j = k + 1; j < n; ++j) {
+            const double f = A[k + (size_t)j * n];
+            if (f == 0.0) continue;
+            for (int i = k + 1; i < n; ++i) A[i + (size_t)j * n] -= A[i + (size_t)k * n] * f;
+        }
+    }
+    for (int r = 0; r < nrhs; ++r) {
+        double* b = B + (size_t)r * n;
+        for (int k = 0; k < n; ++k) if (p[k] != k) std::swap(b[k], b[p[k]]);
+        for (int k = 0; k < n; ++k) for (int i = k + 1; i < n; ++i) b[i] -= A[i + (size_t)k * n] * b[k];
+        for (int k = n - 1; k >= 0; --k) {
+            b[k] /= A[k + (size_t)k * n];
+            for (int i = 0; i < k; ++i) b[i] -= A[i + (size_t)k * n] * b[k];
+        }
+    }
+    return true;
+}
+
+}  // namespace
+
+int phi1_pade(double* phiH, const double* H, double alpha, int m, int ldh, int ldphi)
+{
+    if (m == 1) {
+        const double a = alpha * H[0];
+        phiH[0] = (a == 0.0) ? 1.0 : std::expm1(a) / a;
+        return 0;
+    }
+    static const double num[6] = {1.0 / 26.0, 5.0 / 156.0, 1.0 / 858.0, 1.0 / 5720.0, 1.0 / 205920.0, 1.0 / 8648640.0};
+    static const double den[6] = {-6.0 / 13.0, 5.0 / 52.0, -5.0 / 429.0, 1.0 / 1144.0, -1.0 / 25740.0, 1.0 / 1235520.0};
+    const size_t n2 = (size_t)m * m;
+    std::vector<double> buf(6 * n2);
+    double *A = buf.data(), *P = A + n2, *N = P + n2, *D = N + n2, *E = D + n2, *T = E + n2;
+    for (int j = 0; j < m; ++j)
+        for (int i = 0; i < m; ++i) A[i + (size_t)j * m] = alpha * H[i + (size_t)j * ldh];
+    const double na = inf_norm(m, A);
+    int s = na > 0.0 ? (int)std::ceil(1.0 + std::log(na) / std::log(2.0)) : 0;
+    if (s < 0) s = 0;
+    const double sc = 1.0 / std::pow(2.0, (double)s);
+    for (size_t i = 0; i < n2; ++i) { A[i] *= sc; P[i] = A[i]; N[i] = 0.0; D[i] = 0.0; E[i] = 0.0; }
+    for (int i = 0; i < m; ++i) { N[i + (size_t)i * m] = 1.0; D[i + (size_t)i * m] = 1.0; E[i + (size_t)i * m] = 1.0; }
+    std::vector<double> tmp(n2);
+    for (int k = 0; k < 6; ++k) {
+        if (k > 0) { mm(m, 1.0, P, A, 0.0, T, tmp.data()); std::memcpy(P, T, n2 * sizeof(double)); }
+        for (size_t i = 0; i < n2; ++i) { N[i] += num[k] * P[i]; D[i] += den[k] * P[i]; }
+    }
+    if (!solve(m, D, N, m)) return 1;
+    if (s > 0) {
+        mm(m, 1.0, A, N, 1.0, E, tmp.data());
+        std::memcpy(T, N, n2 * sizeof(double));
+        mm(m, 0.5, E, N, 0.5, T, tmp.data());
+        std::memcpy(N, T, n2 * sizeof(double));
+        for (int k = 1; k < s; ++k) {
+            mm(m, 1.0, E, E, 0.0, T, tmp.data());
+            std::memcpy(E, T, n2 * sizeof(double));
+            std::memcpy(T, N, n2 * sizeof(double));
+            mm(m, 0.5, E, N, 0.5, T, tmp.data());
+            std::memcpy(N, T, n2 * sizeof(double));
+        }
+    }
+    for (int j = 0; j < m; ++j)
+        for (int i = 0; i < m; ++i) phiH[i + (size_t)j * ldphi] = N[i + (size_t)j * m];
+    return 0;
+}
+
+int expm_pade6(double* E, const double* A0, int n)
+{
+    double c[7];
+    c[0] = 1.0;
+    for (int k = 1; k <= 6; ++k) c[k] = c[k - 1] * (double)(7 - k) / (double)(k * (13 - k));
+    const size_t n2 = (size_t)n * n;
+    std::vector<double> buf(5 * n2);
+    double *A = buf.data(), *P = A + n2, *N = P + n2, *D = N + n2, *T = D + n2;
+    std::vector<double> tmp(n2);
+    std::memcpy(A, A0, n2 * sizeof(double));
+    const double na = inf_norm(n, A);
+    int s = 0;
+    if (na > 0.5) s = (int)std::ceil(std::log2(na / 0.5));
+    const double sc = std::ldexp(1.0, -s);
+    for (size_t i = 0; i < n2; ++i) { A[i] *= sc; P[i] = A[i]; N[i] = 0.0; D[i] = 0.0; }
+    for (int i = 0; i < n; ++i) { N[i + (size_t)i * n] = 1.0; D[i + (size_t)i * n] = 1.0; }
+    for (int k = 1; k <= 6; ++k) {
+        if (k > 1) { mm(n, 1.0, P, A, 0.0, T, tmp.data()); std::memcpy(P, T, n2 * sizeof(double)); }
+        const double sg = (k & 1) ? -1.0 : 1.0;
+        for (size_t i = 0; i < n2; ++i) { N[i] += c[k] * P[i]; D[i] += sg * c[k] * P[i]; }
+    }
+    if (!solve(n, D, N, n)) return 1;
+    for (int k = 0; k < s; ++k) { mm(n, 1.0, N, N, 0.0, T, tmp.data()); std::memcpy(N, T, n2 * sizeof(double)); }
+    std::memcpy(E, N, n2 * sizeof(double));
+    return 0;
+}
+
+// out[(k-1)*m + i] = [phi_k(tau H) e1]_i for k = 1..kmax (Saad 1992; Sidje 1998, Thm 1).
+int phik_e1(double* out, const double* H, int ldh, int m, double tau, int kmax)
+{
+    const int n = m + kmax;
+    std::vector<double> A((size_t)n * n, 0.0), E((size_t)n * n);
+    for (int j = 0; j < m; ++j)
+        for (int i = 0; i < m; ++i) A[i + (size_t)j * n] = tau * H[i + (size_t)j * ldh];
+    A[(size_t)m * n] = 1.0;
+    for (int k = 1; k < kmax; ++k) A[(m + k - 1) + (size_t)(m + k) * n] = 1.0;
+    if (expm_pade6(E.data(), A.data(), n)) return 1;
+    for (int k = 1; k <= kmax; ++k)
+        for (int i = 0; i < m; ++i) out[(size_t)(k - 1) * m + i] = E[i + (size_t)(m + k - 1) * n];
+    return 0;
+}
+
+}  // namespace tsr
+
+extern "C" int phipade(double* phiH, double* H, double alpha, int m, int MaxKrylov)
+{
+    return tsr::phi1_pade(phiH, H, alpha, m, MaxKrylov + 1, MaxKrylov);
+}

@@ -1,0 +1,142 @@
+// device.h - device-side problem layout shared by the HIP sources.
+//
+// HBM layout: one allocation per problem holding every template's geometry (SoA,
+// the reference's element-major order), every scenario's macro mesh and Keff table,
+// the per-cell tables, and G's scratch (macro interpolant coefficients, macro
+// sub-face fluxes, BC half-edge inflows, per-cell exchange).  The unknown vector
+// itself is [scenario 0: macro nodes | cells ...][scenario 1: ...] in FP64.
+#ifndef TSR_DEVICE_H
+#define TSR_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include "../../include/tsr/tsr_types.h"
+
+namespace tsr {
+
+constexpr int kMicroBS = 256;         // threads per micro-cell workgroup
+constexpr int kMicroNPT = 4;          // nodes per thread: templates up to 1024 nodes
+constexpr int kMaxNodeInc = 12;       // macro elements around a node hosting a cell
+constexpr int kVecBS = 256;           // threads per workgroup of the vector kernels
+constexpr int kVecR = 4;              // rows per thread of the vector kernels
+constexpr int kMaxCols = 128;         // Krylov basis columns a pass can read
+
+struct DSoil { double Ks, tr, ts, al, n, m; };
+
+struct DTmpl {
+    int nn, ne;
+    const int* ptr;                   // [nn+1] node -> incident (e*4 + lv), e ascending
+    const int* inc;
+    const int* el;                    // [3*ne]
+    const double* sf;                 // [9*ne] shape-function coefficients (rows 0..5 used)
+    const double* nrm;                // [6*ne]
+    const double* w;                  // [9*ne] (j*3+v)*ne + e: nodal-K weight of vertex v at sub-face j
+    const double* scv;                // [ne] SCV area of each vertex (|det|/6)
+    const double* vec;                // [2*nn]
+    const double* cen;                // [2*ne]
+    const int* bnd;                   // [nn]
+    const unsigned char* soil;        // [ne] or nullptr
+    double cell_area;
+};
+
+struct DScen {
+    DSoil A, B;
+    double epsA, dirichlet_head, boundary_flux;
+    double keff_logr, keff_first;
+    double box[4];                    // bounding box of the macro nodes (xmin, xmax, ymin, ymax)
+    int model, host, nv, nn, ne, n_cells, first_cell, keff_n;
+    long long u_off;
+    const double* keff;               // [4*n]
+    const int* el;
+    const double* nodes;
+    const double* sf;
+    const double* nrm;
+    const double* midx;
+    const double* midy;
+    const double* cv;
+    const double* scv;
+    const int* bce;
+    const double* be_len;
+    const double* be_mid;
+    const double* be_nrm;
+    const double* cen;
+    const int* ptr;
+    const int* inc;
+    const unsigned char* dir;
+    double* alpha;                    // [4*ne] scratch
+    double* qe;                       // [nv*ne] scratch
+    double* bq;                       // [2*nv*ne] scratch
+};
+
+struct DBatch {
+    int n_scen, n_tmpl, n_cells, max_discs;
+    int nn_max, ne_max, macro_nn_max, macro_ne_max;
+    const DScen* scen;
+    const DTmpl* tmpl;
+    const int* cell_tmpl;
+    const int* cell_scen;
+    const long long* cell_off;
+    const double* discs;
+    double* Q;                        // [n_cells]
+    int* bad;                         // [n_scen] set when G produced a non-finite value
+    double Ss;
+    long long N;
+};
+
+// Per-scenario perturbation / finite-difference arguments of one G launch:
+//   state read  = u + veps[s] * v        (v may be null)
+//   value written = (G - g0) * ieps[s]   (g0 may be null)
+struct GArgs {
+    const double* u;
+    const double* v;
+    const double* veps;
+    const double* g0;
+    const double* ieps;
+    const int* active;                // scenario mask (nullptr = all)
+    double* out;
+};
+
+struct DevProblem;
+DevProblem* upload_problem(const tsr_batch_view& v, std::string& err);
+void        free_problem(DevProblem* p);
+const DBatch& batch_of(const DevProblem* p);
+size_t      micro_lds_bytes(const DevProblem* p);
+hipError_t  launch_gfunc(const DevProblem* p, const GArgs& a, hipStream_t st);
+
+// Row blocks of the vector kernels: block b covers rows [r0, r1) of scenario scen.
+struct BlkMap {
+    const int* scen;
+    const long long* r0;
+    const long long* r1;
+};
+
+enum PassMode { PASS_DOTS = 0, PASS_AXPY_DOTS = 1, PASS_AXPY_NORM = 2 };
+enum RedMode { RED_H1 = 0, RED_H2 = 1, RED_NORM = 2, RED_START = 3, RED_SUM = 4 };
+
+struct KrylovState {              // per-scenario device arrays, stride ld
+    double* cs;                   // column scales: v_k = cs[k] * raw column k
+    double* coef;                 // coefficients of the current axpy
+    double* H;                    // Hessenberg, column-major (ldh = mmax+1) per scenario
+    double* eps;                  // FD step
+    double* veps;                 // eps * cs[j] for the next G call
+    double* ieps;                 // 1 / eps
+    double* res;                  // misc results per scenario (8 slots)
+    int ld, ldh, hstride;
+};
+
+hipError_t launch_pass(int mode, const double* const* cols, int nk, double* w, const double* u, const double* coef,
+                       int ldc, const BlkMap& M, int nblk, double* part, int ldp, const int* active, double at,
+                       double rt, hipStream_t st);
+hipError_t launch_norms(const double* a, const double* u, const BlkMap& M, int nblk, double* part, int ldp,
+                        const int* active, double at, double rt, hipStream_t st);
+hipError_t launch_lincomb(double* out, const double* base, const double* const* cols, const int* nk_s,
+                          const double* coef, int ldc, double sgn, const BlkMap& M, int nblk, double* part, int ldp,
+                          int partmode, const double* u, double at, double rt, const double* extra_sub,
+                          const int* active, hipStream_t st);
+hipError_t launch_reduce(int mode, const double* part, int ldp, int nk, const int* blk_first, int n_scen,
+                         const long long* scen_rows, KrylovState K, int j, const int* active, hipStream_t st);
+hipError_t launch_masked_copy(double* dst, const double* src, const BlkMap& M, int nblk, const int* mask,
+                              hipStream_t st);
+
+}  // namespace tsr
+
+#endif

@@ -1,0 +1,623 @@
+// gfunc.hip - the two-scale RHS G(u) on CDNA4 (gfx950), FP64.
+//
+// One G evaluation is three launches on one stream:
+//   k_macro_elem  one thread per macro element: macro interpolant coefficients, the
+//                 Keff-weighted sub-face fluxes and the BC half-edge inflows;
+//   k_micro       one workgroup per micro cell: node heads, element soils and the
+//                 node-centric van Genuchten evaluation in LDS, sub-face fluxes in
+//                 LDS, a deterministic node-centric gather, and the exchange term
+//                 (net inflow into the cell's non-Dirichlet CVs) by a wavefront
+//                 shuffle reduction;
+//   k_macro_node  one thread per macro node: gather of sub-face fluxes, BC inflows
+//                 and the exchange, divided by the macro storage.
+// The discrete model (geometry conventions, unknown layout, micro Dirichlet heads,
+// macro storage) follows the reference's data structures: main.cpp:260,
+// save_solution.cpp:40-101, micro_mesh_properties.cpp:259-351,
+// macro_mesh_properties.cpp:289-363, read_effective_conductivity.cpp:44-58, Keff.h:3.
+// Every kernel reads the state as u + veps*v (the finite-difference perturbation of
+// the Krylov iteration, never materialised) and can write (G - g0) * ieps directly.
+#include "device.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace tsr {
+
+// ----------------------------------------------------------------------------------
+// van Genuchten (1980) / Mualem (1976, l = 1/2) with specific storage Ss.
+// C = (ts-tr) m n x Se / (|h| (1+x)) + Ss,  K = Ks sqrt(Se) (1 - (x/(1+x))^m)^2,
+// x = (alpha|h|)^n, Se = (1+x)^-m; 1 - (x/(1+x))^m = -expm1(-m log1p(1/x)).
+// ----------------------------------------------------------------------------------
+__device__ __forceinline__ void vg_ck(const DSoil& s, double h, double Ss, double& C, double& K)
+{
+    if (h < 0.0) {
+        const double L = log(s.al * (-h));
+        const double x = exp(s.n * L);
+        const double P = log1p(x);
+        const double Se = exp(-s.m * P);
+        C = (s.ts - s.tr) * s.m * s.n * x * Se / ((-h) * (1.0 + x)) + Ss;
+        const double B = -expm1(-s.m * log1p(1.0 / x));
+        K = s.Ks * sqrt(Se) * B * B;
+    } else {
+        C = Ss;
+        K = s.Ks;
+    }
+}
+
+__device__ __forceinline__ void keff_dev(const DScen& S, double h, double K[4])
+{
+    const int n = S.keff_n;
+    const double lh = (h < 0.0) ? log10(-h) : -INFINITY;
+    double tt = (lh - S.keff_first) / S.keff_logr;
+    tt = fmin(fmax(tt, 0.0), (double)(n - 1));
+    int i = (int)tt;
+    if (i > n - 2) i = n - 2;
+    const double fr = tt - (double)i;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const double a = S.keff[c * n + i], b = S.keff[c * n + i + 1];
+        K[c] = a + fr * (b - a);
+    }
+}
+
+__device__ __forceinline__ double ld_state(const GArgs& a, double ve, long long i)
+{
+    double x = a.u[i];
+    if (a.v) x += ve * a.v[i];
+    return x;
+}
+
+__device__ __forceinline__ double macro_head(const DScen& S, const GArgs& a, double ve, int j)
+{
+    return S.dir[j] ? S.dirichlet_head : ld_state(a, ve, S.u_off + j);
+}
+
+__device__ __forceinline__ double wave_sum(double x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+__device__ __forceinline__ void store_out(const DBatch& B, const GArgs& a, int s, long long i, double g)
+{
+    if (a.g0) g = (g - a.g0[i]) * a.ieps[s];
+    if (!isfinite(g)) atomicOr(B.bad + s, 1);
+    a.out[i] = g;
+}
+
+// ---------------------------------------------------------------- macro elements
+__global__ __launch_bounds__(256) void k_macro_elem(DBatch B, GArgs a)
+{
+    const int s = blockIdx.y;
+    if (a.active && !a.active[s]) return;
+    const DScen& S = B.scen[s];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= S.ne) return;
+    const int nv = S.nv, ne = S.ne;
+    const double ve = a.v ? a.veps[s] : 0.0;
+    double hv[4] = {0, 0, 0, 0}, al[4] = {0, 0, 0, 0};
+    for (int i = 0; i < nv; ++i) hv[i] = macro_head(S, a, ve, S.el[i * ne + e]);
+    for (int k = 0; k < nv; ++k) {
+        double acc = 0.0;
+        for (int i = 0; i < nv; ++i) acc += hv[i] * S.sf[(k * nv + i) * ne + e];
+        al[k] = acc;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) S.alpha[4 * e + k] = al[k];
+    for (int j = 0; j < nv; ++j) {
+        const double x = S.midx[j * ne + e], y = S.midy[j * ne + e];
+        double gx, gy, hip;
+        if (nv == 3) { gx = al[0]; gy = al[1]; hip = al[0] * x + al[1] * y + al[2]; }
+        else { gx = al[0] + al[2] * y; gy = al[1] + al[2] * x; hip = al[0] * x + al[1] * y + al[2] * x * y + al[3]; }
+        double K[4];
+        keff_dev(S, hip, K);
+        const double nx = S.nrm[(2 * j) * ne + e], ny = S.nrm[(2 * j + 1) * ne + e];
+        S.qe[e * nv + j] = -((K[0] * gx + K[1] * (gy + 1.0)) * nx + (K[2] * gx + K[3] * (gy + 1.0)) * ny);
+    }
+    for (int p = 0; p < nv; ++p) {
+        const int type = S.bce[p * ne + e];
+        for (int hk = 0; hk < 2; ++hk) {
+            double f = 0.0;
+            if (type == TSR_BC_FLUX) {
+                f = S.boundary_flux * S.be_len[(2 * p + hk) * ne + e];
+            } else if (type == TSR_BC_DHDX0 || type == TSR_BC_DHDY0) {
+                const double bx = S.be_mid[(4 * p + 2 * hk) * ne + e], by = S.be_mid[(4 * p + 2 * hk + 1) * ne + e];
+                const double bnx = S.be_nrm[(4 * p + 2 * hk) * ne + e], bny = S.be_nrm[(4 * p + 2 * hk + 1) * ne + e];
+                double gx, gy, hip;
+                if (nv == 3) { gx = al[0]; gy = al[1]; hip = al[0] * bx + al[1] * by + al[2]; }
+                else { gx = al[0] + al[2] * by; gy = al[1] + al[2] * bx; hip = al[0] * bx + al[1] * by + al[2] * bx * by + al[3]; }
+                if (type == TSR_BC_DHDX0) gx = 0.0; else gy = 0.0;
+                double K[4];
+                keff_dev(S, hip, K);
+                f = (K[0] * gx + K[1] * (gy + 1.0)) * bnx + (K[2] * gx + K[3] * (gy + 1.0)) * bny;
+            }
+            S.bq[e * 2 * nv + 2 * p + hk] = f;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- micro cells
+__device__ __forceinline__ int elem_soil(const DBatch& B, const DTmpl& T, int cell, int e)
+{
+    if (B.max_discs > 0) {
+        const double cx = T.cen[e], cy = T.cen[T.ne + e];
+        const double* d = B.discs + (size_t)cell * B.max_discs * 3;
+        for (int k = 0; k < B.max_discs; ++k) {
+            const double r = d[3 * k + 2];
+            if (r > 0.0) {
+                const double dx = cx - d[3 * k], dy = cy - d[3 * k + 1];
+                if (dx * dx + dy * dy < r * r) return TSR_SOIL_B;
+            }
+        }
+        return TSR_SOIL_A;
+    }
+    return T.soil ? (int)T.soil[e] : TSR_SOIL_B;
+}
+
+__global__ __launch_bounds__(kMicroBS) void k_micro(DBatch B, GArgs a)
+{
+    extern __shared__ __align__(16) double smem[];
+    __shared__ double s_bc[6];
+    __shared__ double s_mel[kMaxNodeInc][12];
+    __shared__ int s_ninc;
+    __shared__ double s_red[kMicroBS / 64];
+
+    const int cell = blockIdx.x;
+    const int s = B.cell_scen[cell];
+    if (a.active && !a.active[s]) return;
+    const int t = B.cell_tmpl[cell];
+    const DTmpl& T = B.tmpl[t];
+    const DScen& S = B.scen[s];
+    const int nn = T.nn, ne = T.ne, tid = threadIdx.x;
+    double* h = smem;
+    double* KA = h + B.nn_max;
+    double* KB = KA + B.nn_max;
+    double* q = KB + B.nn_max;
+    unsigned char* soil = reinterpret_cast<unsigned char*>(q + 3 * B.ne_max);
+    const double ve = a.v ? a.veps[s] : 0.0;
+    const long long off = B.cell_off[cell];
+    const int host = cell - S.first_cell;
+    const int model = S.model;
+    const bool node_host = (S.host == TSR_HOST_NODE);
+
+    // host interpolation data
+    if (!node_host) {
+        if (tid == 0) {
+            const double* al = S.alpha + 4 * host;
+            if (S.nv == 3) { s_bc[0] = al[0]; s_bc[1] = al[1]; s_bc[2] = 0.0; s_bc[3] = al[2]; }
+            else { s_bc[0] = al[0]; s_bc[1] = al[1]; s_bc[2] = al[2]; s_bc[3] = al[3]; }
+            s_bc[4] = S.cen[host];
+            s_bc[5] = S.cen[S.ne + host];
+        }
+    } else {
+        const int q0 = S.ptr[host], ninc = S.ptr[host + 1] - q0;
+        if (tid == 0) {
+            s_ninc = ninc;
+            s_bc[3] = macro_head(S, a, ve, host);
+            s_bc[4] = S.nodes[host];
+            s_bc[5] = S.nodes[S.nn + host];
+        }
+        if (model != 1 && tid < ninc) {
+            const int e = S.inc[q0 + tid] >> 2;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) s_mel[tid][k] = S.sf[k * S.ne + e];
+#pragma unroll
+            for (int v = 0; v < 3; ++v) s_mel[tid][9 + v] = macro_head(S, a, ve, S.el[v * S.ne + e]);
+        }
+    }
+    __syncthreads();
+
+    // heads (Dirichlet nodes from the macro field) and element soils
+    for (int i = tid; i < nn; i += kMicroBS) {
+        double hi;
+        if (T.bnd[i] == 1) {
+            if (node_host) {
+                if (model == 1) {
+                    hi = s_bc[3];
+                } else {
+                    const double px = fmin(fmax(s_bc[4] + T.vec[i], S.box[0]), S.box[1]);
+                    const double py = fmin(fmax(s_bc[5] + T.vec[nn + i], S.box[2]), S.box[3]);
+                    int best = -1;
+                    double bmin = -INFINITY, lam[3] = {0, 0, 0};
+                    for (int k = 0; k < s_ninc; ++k) {
+                        double l[3], mn = INFINITY;
+#pragma unroll
+                        for (int v = 0; v < 3; ++v) {
+                            l[v] = s_mel[k][v] * px + s_mel[k][3 + v] * py + s_mel[k][6 + v];
+                            mn = fmin(mn, l[v]);
+                        }
+                        if (best < 0 || mn > bmin + 1e-12) { bmin = mn; best = k; lam[0] = l[0]; lam[1] = l[1]; lam[2] = l[2]; }
+                    }
+                    double ws = 0.0, hs = 0.0;
+#pragma unroll
+                    for (int v = 0; v < 3; ++v) {
+                        const double w = fmax(lam[v], 0.0);
+                        ws += w;
+                        hs += w * s_mel[best][9 + v];
+                    }
+                    hi = hs / ws;
+                }
+            } else {
+                const double X = (model == 1) ? s_bc[4] : s_bc[4] + T.vec[i];
+                const double Y = (model == 1) ? s_bc[5] : s_bc[5] + T.vec[nn + i];
+                hi = s_bc[0] * X + s_bc[1] * Y + s_bc[2] * X * Y + s_bc[3];
+            }
+        } else {
+            hi = ld_state(a, ve, off + i);
+        }
+        h[i] = hi;
+    }
+    for (int e = tid; e < ne; e += kMicroBS) soil[e] = (unsigned char)elem_soil(B, T, cell, e);
+    __syncthreads();
+
+    // node-centric constitutive evaluation: storage kept in registers for the gather
+    double stor[kMicroNPT];
+#pragma unroll
+    for (int k = 0; k < kMicroNPT; ++k) {
+        stor[k] = 1.0;
+        const int i = tid + k * kMicroBS;
+        if (i < nn) {
+            double cva = 0.0, cvb = 0.0;
+            for (int qq = T.ptr[i]; qq < T.ptr[i + 1]; ++qq) {
+                const int e = T.inc[qq] >> 2;
+                if (soil[e] == TSR_SOIL_A) cva += T.scv[e]; else cvb += T.scv[e];
+            }
+            double cA = 0.0, cB = 0.0, kA = 0.0, kB = 0.0;
+            // first evaluation with the lane's own soil (no divergence), second only at interfaces
+            const bool hasA = cva > 0.0, hasB = cvb > 0.0;
+            {
+                double C, K;
+                vg_ck(hasA ? S.A : S.B, h[i], B.Ss, C, K);
+                if (hasA) { cA = C; kA = K; } else { cB = C; kB = K; }
+            }
+            if (hasA && hasB) {
+                double C, K;
+                vg_ck(S.B, h[i], B.Ss, C, K);
+                cB = C; kB = K;
+            }
+            KA[i] = kA;
+            KB[i] = kB;
+            stor[k] = cA * cva + cB * cvb;
+        }
+    }
+    __syncthreads();
+
+    // sub-face fluxes: q[e*3+j] leaves v_j, enters v_{j+1}
+    for (int e = tid; e < ne; e += kMicroBS) {
+        const int v0 = T.el[e], v1 = T.el[ne + e], v2 = T.el[2 * ne + e];
+        const double h0 = h[v0], h1 = h[v1], h2 = h[v2];
+        const double gx = T.sf[e] * h0 + T.sf[ne + e] * h1 + T.sf[2 * ne + e] * h2;
+        const double gy = T.sf[3 * ne + e] * h0 + T.sf[4 * ne + e] * h1 + T.sf[5 * ne + e] * h2;
+        const double* Ks = (soil[e] == TSR_SOIL_A) ? KA : KB;
+        const double k0 = Ks[v0], k1 = Ks[v1], k2 = Ks[v2];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double Kf = T.w[(j * 3) * ne + e] * k0 + T.w[(j * 3 + 1) * ne + e] * k1 + T.w[(j * 3 + 2) * ne + e] * k2;
+            const double nx = T.nrm[(2 * j) * ne + e], ny = T.nrm[(2 * j + 1) * ne + e];
+            q[e * 3 + j] = -Kf * (gx * nx + (gy + 1.0) * ny);
+        }
+    }
+    __syncthreads();
+
+    // node-centric gather (incident order = element order, identical on every run)
+    double Qacc = 0.0;
+#pragma unroll
+    for (int k = 0; k < kMicroNPT; ++k) {
+        const int i = tid + k * kMicroBS;
+        if (i < nn) {
+            double net = 0.0;
+            for (int qq = T.ptr[i]; qq < T.ptr[i + 1]; ++qq) {
+                const int e = T.inc[qq] >> 2, lv = T.inc[qq] & 3;
+                net += q[e * 3 + (lv + 2) % 3];
+                net -= q[e * 3 + lv];
+            }
+            double gi;
+            if (T.bnd[i] == 1) gi = 0.0;
+            else { gi = net / stor[k]; Qacc += net; }
+            store_out(B, a, s, off + i, gi);
+        }
+    }
+    Qacc = wave_sum(Qacc);
+    if ((tid & 63) == 0) s_red[tid >> 6] = Qacc;
+    __syncthreads();
+    if (tid == 0) {
+        double Q = 0.0;
+        for (int w = 0; w < kMicroBS / 64; ++w) Q += s_red[w];
+        B.Q[cell] = Q;
+    }
+}
+
+// ---------------------------------------------------------------- macro nodes
+__global__ __launch_bounds__(256) void k_macro_node(DBatch B, GArgs a)
+{
+    const int s = blockIdx.y;
+    if (a.active && !a.active[s]) return;
+    const DScen& S = B.scen[s];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= S.nn) return;
+    const int nv = S.nv, ne = S.ne;
+    double net = 0.0;
+    for (int qq = S.ptr[j]; qq < S.ptr[j + 1]; ++qq) {
+        const int e = S.inc[qq] >> 2, lv = S.inc[qq] & 3;
+        const int prev = (lv + nv - 1) % nv;
+        net += S.qe[e * nv + prev];
+        net -= S.qe[e * nv + lv];
+        net += S.bq[e * 2 * nv + 2 * lv];
+        net += S.bq[e * 2 * nv + 2 * prev + 1];
+    }
+    if (S.model == 3) {
+        if (S.host == TSR_HOST_ELEMENT) {
+            for (int qq = S.ptr[j]; qq < S.ptr[j + 1]; ++qq) {
+                const int e = S.inc[qq] >> 2, lv = S.inc[qq] & 3;
+                const int cell = S.first_cell + e;
+                net -= B.Q[cell] * S.scv[lv * ne + e] / B.tmpl[B.cell_tmpl[cell]].cell_area;
+            }
+        } else {
+            const int cell = S.first_cell + j;
+            net -= B.Q[cell] * (S.cv[j] / B.tmpl[B.cell_tmpl[cell]].cell_area);
+        }
+    }
+    double g = 0.0;
+    if (!S.dir[j]) {
+        const double ve = a.v ? a.veps[s] : 0.0;
+        double C, K;
+        vg_ck(S.A, ld_state(a, ve, S.u_off + j), B.Ss, C, K);
+        g = net / (S.epsA * C * S.cv[j]);
+    }
+    store_out(B, a, s, S.u_off + j, g);
+}
+
+// ---------------------------------------------------------------- upload
+struct DevProblem {
+    void* buf = nullptr;              // one HBM allocation
+    size_t bytes = 0;
+    DBatch B{};
+    size_t lds = 0;
+};
+
+namespace {
+
+struct Arena {
+    std::vector<unsigned char> host;
+    template <class T> size_t put(const T* src, size_t n)
+    {
+        size_t off = (host.size() + 255) & ~size_t(255);
+        host.resize(off + sizeof(T) * n);
+        if (n) std::memcpy(host.data() + off, src, sizeof(T) * n);
+        return off;
+    }
+    template <class T> size_t reserve(size_t n)
+    {
+        size_t off = (host.size() + 255) & ~size_t(255);
+        host.resize(off + sizeof(T) * n, 0);
+        return off;
+    }
+};
+
+template <class T> T* at(void* base, size_t off) { return reinterpret_cast<T*>(static_cast<unsigned char*>(base) + off); }
+
+void csr(int nn, int ne, int nv, const int* el, std::vector<int>& ptr, std::vector<int>& inc)
+{
+    std::vector<int> cnt(nn + 1, 0);
+    for (int e = 0; e < ne; ++e)
+        for (int lv = 0; lv < nv; ++lv) cnt[el[lv * ne + e] + 1]++;
+    ptr.assign(nn + 1, 0);
+    for (int i = 0; i < nn; ++i) ptr[i + 1] = ptr[i] + cnt[i + 1];
+    inc.assign(ptr[nn], 0);
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int e = 0; e < ne; ++e)
+        for (int lv = 0; lv < nv; ++lv) inc[fill[el[lv * ne + e]]++] = e * 4 + lv;
+}
+
+}  // namespace
+
+DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
+{
+    Arena A;
+    struct TOff { size_t ptr, inc, el, sf, nrm, w, scv, vec, cen, bnd, soil; int nn, ne; double ca; bool has_soil; };
+    struct SOff { size_t keff, el, nodes, sf, nrm, midx, midy, cv, scv, bce, blen, bmid, bnrm, cen, ptr, inc, dir, alpha, qe, bq; };
+    std::vector<TOff> to(v.n_tmpl);
+    std::vector<SOff> so(v.n_scen);
+    int nn_max = 0, ne_max = 0, mnn = 0, mne = 0;
+    for (int t = 0; t < v.n_tmpl; ++t) {
+        const micro_mesh_struct& m = v.tmpl[t];
+        if (m.no_verts != 3) { err = "micro templates must be triangles"; return nullptr; }
+        const int nn = m.no_nodes, ne = m.no_elements;
+        if (nn > kMicroBS * kMicroNPT) { err = "micro template has more than 1024 nodes"; return nullptr; }
+        nn_max = std::max(nn_max, nn);
+        ne_max = std::max(ne_max, ne);
+        std::vector<int> ptr, inc;
+        csr(nn, ne, 3, m.elements, ptr, inc);
+        std::vector<double> w((size_t)9 * ne), scv(ne);
+        for (int e = 0; e < ne; ++e) {
+            const int a0 = m.elements[e], b0 = m.elements[ne + e], c0 = m.elements[2 * ne + e];
+            const double xa = m.nodes[a0], ya = m.nodes[nn + a0];
+            const double xb = m.nodes[b0], yb = m.nodes[nn + b0];
+            const double xc = m.nodes[c0], yc = m.nodes[nn + c0];
+            const double det = (xb - xa) * (yc - ya) - (xc - xa) * (yb - ya);
+            scv[e] = std::fabs(det) / 6.0;
+            for (int j = 0; j < 3; ++j) {
+                const double mx = m.midx[j * ne + e], my = m.midy[j * ne + e];
+                for (int vv = 0; vv < 3; ++vv)
+                    w[(size_t)(j * 3 + vv) * ne + e] = m.shape_funcs[(0 * 3 + vv) * ne + e] * mx +
+                                                       m.shape_funcs[(1 * 3 + vv) * ne + e] * my +
+                                                       m.shape_funcs[(2 * 3 + vv) * ne + e];
+            }
+        }
+        TOff& o = to[t];
+        o.nn = nn; o.ne = ne; o.ca = m.cell_area;
+        o.ptr = A.put(ptr.data(), ptr.size());
+        o.inc = A.put(inc.data(), inc.size());
+        o.el = A.put(m.elements, (size_t)3 * ne);
+        o.sf = A.put(m.shape_funcs, (size_t)9 * ne);
+        o.nrm = A.put(m.normals, (size_t)6 * ne);
+        o.w = A.put(w.data(), w.size());
+        o.scv = A.put(scv.data(), scv.size());
+        o.vec = A.put(m.vectors, (size_t)2 * nn);
+        o.cen = A.put(m.centroid_elements, (size_t)2 * ne);
+        o.bnd = A.put(m.boundary_nodes, (size_t)nn);
+        o.has_soil = v.tmpl_soil && v.tmpl_soil[t];
+        o.soil = o.has_soil ? A.put(v.tmpl_soil[t], (size_t)ne) : 0;
+    }
+    // scenarios sharing one macro mesh share its device arrays
+    std::vector<int> share(v.n_scen, -1);
+    for (int s = 0; s < v.n_scen; ++s) {
+        const tsr_scenario& sc = v.scen[s];
+        const macro_mesh_struct& M = sc.ud.macro_mesh;
+        const Keff_struct& K = sc.ud.effective_conductivity;
+        const int nv = M.no_verts, nn = M.no_nodes, ne = M.no_elements;
+        mnn = std::max(mnn, nn);
+        mne = std::max(mne, ne);
+        SOff& o = so[s];
+        o.keff = A.put(K.Keff_values, (size_t)4 * K.no_hvalues);
+        for (int r = 0; r < s; ++r)
+            if (v.scen[r].ud.macro_mesh.elements == M.elements && v.scen[r].ud.macro_mesh.no_nodes == nn) { share[s] = r; break; }
+        if (sc.host_kind == TSR_HOST_NODE && nv != 3) { err = "cells hosted by macro nodes need a triangle macro mesh"; return nullptr; }
+        if (share[s] < 0) {
+            std::vector<int> ptr, inc;
+            csr(nn, ne, nv, M.elements, ptr, inc);
+            for (int j = 0; j < nn; ++j)
+                if (sc.host_kind == TSR_HOST_NODE && ptr[j + 1] - ptr[j] > kMaxNodeInc) { err = "macro node with too many elements"; return nullptr; }
+            std::vector<unsigned char> dir(nn, 0);
+            for (int e = 0; e < ne; ++e)
+                for (int p = 0; p < nv; ++p)
+                    if (M.boundary_condition_edges[p * ne + e] == TSR_BC_DIRICHLET) {
+                        dir[M.elements[p * ne + e]] = 1;
+                        dir[M.elements[((p + 1) % nv) * ne + e]] = 1;
+                    }
+            o.el = A.put(M.elements, (size_t)nv * ne);
+            o.nodes = A.put(M.nodes, (size_t)2 * nn);
+            o.sf = A.put(M.shape_funcs, (size_t)nv * nv * ne);
+            o.nrm = A.put(M.normals, (size_t)2 * nv * ne);
+            o.midx = A.put(M.midx, (size_t)nv * ne);
+            o.midy = A.put(M.midy, (size_t)nv * ne);
+            o.cv = A.put(M.CV_area, (size_t)nn);
+            o.scv = A.put(M.SCV_area, (size_t)nv * ne);
+            o.bce = A.put(M.boundary_condition_edges, (size_t)nv * ne);
+            o.blen = A.put(M.boundary_edge_lens, (size_t)2 * nv * ne);
+            o.bmid = A.put(M.boundary_edge_midpts, (size_t)4 * nv * ne);
+            o.bnrm = A.put(M.boundary_edge_normals, (size_t)4 * nv * ne);
+            o.cen = A.put(M.centroid_elements, (size_t)2 * ne);
+            o.ptr = A.put(ptr.data(), ptr.size());
+            o.inc = A.put(inc.data(), inc.size());
+            o.dir = A.put(dir.data(), dir.size());
+        } else {
+            const SOff& r = so[share[s]];
+            const size_t keep = o.keff;
+            o = r;
+            o.keff = keep;
+        }
+        o.alpha = A.reserve<double>((size_t)4 * ne);
+        o.qe = A.reserve<double>((size_t)nv * ne);
+        o.bq = A.reserve<double>((size_t)2 * nv * ne);
+    }
+    // cells
+    std::vector<int> cell_scen(v.n_cells);
+    for (int s = 0; s < v.n_scen; ++s)
+        for (int c = 0; c < v.scen[s].n_cells; ++c) cell_scen[v.scen[s].first_cell + c] = s;
+    const size_t o_ct = A.put(v.cell_tmpl, (size_t)v.n_cells);
+    const size_t o_cs = A.put(cell_scen.data(), cell_scen.size());
+    const size_t o_co = A.put(v.cell_offset, (size_t)v.n_cells);
+    const size_t o_disc = v.max_discs > 0 ? A.put(v.cell_discs, (size_t)v.n_cells * v.max_discs * 3) : 0;
+    const size_t o_Q = A.reserve<double>((size_t)v.n_cells);
+    const size_t o_bad = A.reserve<int>((size_t)v.n_scen);
+    const size_t o_T = A.reserve<DTmpl>(v.n_tmpl);
+    const size_t o_S = A.reserve<DScen>(v.n_scen);
+
+    DevProblem* P = new DevProblem();
+    P->bytes = A.host.size();
+    if (hipMalloc(&P->buf, P->bytes) != hipSuccess) { err = "hipMalloc failed for the problem"; delete P; return nullptr; }
+    void* base = P->buf;
+    // fill the descriptor tables with device addresses, then copy everything once
+    for (int t = 0; t < v.n_tmpl; ++t) {
+        const TOff& o = to[t];
+        DTmpl T;
+        T.nn = o.nn; T.ne = o.ne; T.cell_area = o.ca;
+        T.ptr = at<int>(base, o.ptr); T.inc = at<int>(base, o.inc); T.el = at<int>(base, o.el);
+        T.sf = at<double>(base, o.sf); T.nrm = at<double>(base, o.nrm); T.w = at<double>(base, o.w);
+        T.scv = at<double>(base, o.scv); T.vec = at<double>(base, o.vec); T.cen = at<double>(base, o.cen);
+        T.bnd = at<int>(base, o.bnd);
+        T.soil = o.has_soil ? at<unsigned char>(base, o.soil) : nullptr;
+        std::memcpy(A.host.data() + o_T + t * sizeof(DTmpl), &T, sizeof(DTmpl));
+    }
+    for (int s = 0; s < v.n_scen; ++s) {
+        const tsr_scenario& sc = v.scen[s];
+        const macro_mesh_struct& M = sc.ud.macro_mesh;
+        const Keff_struct& K = sc.ud.effective_conductivity;
+        const SOff& o = so[s];
+        DScen S;
+        auto soil = [](const soil_struct& x) { return DSoil{x.Ksat, x.thetar, x.thetas, x.alpha, x.n, x.m}; };
+        S.A = soil(sc.ud.soilA); S.B = soil(sc.ud.soilB);
+        S.epsA = K.epsilonA; S.dirichlet_head = sc.dirichlet_head; S.boundary_flux = sc.ud.boundary_flux;
+        S.keff_logr = K.logr; S.keff_first = K.logh_first; S.keff_n = K.no_hvalues;
+        S.box[0] = S.box[2] = INFINITY;
+        S.box[1] = S.box[3] = -INFINITY;
+        for (int i = 0; i < M.no_nodes; ++i) {
+            S.box[0] = std::fmin(S.box[0], M.nodes[i]); S.box[1] = std::fmax(S.box[1], M.nodes[i]);
+            S.box[2] = std::fmin(S.box[2], M.nodes[M.no_nodes + i]); S.box[3] = std::fmax(S.box[3], M.nodes[M.no_nodes + i]);
+        }
+        S.model = sc.ud.model_type; S.host = sc.host_kind; S.nv = M.no_verts; S.nn = M.no_nodes; S.ne = M.no_elements;
+        S.n_cells = sc.n_cells; S.first_cell = sc.first_cell; S.u_off = sc.u_offset;
+        S.keff = at<double>(base, o.keff);
+        S.el = at<int>(base, o.el); S.nodes = at<double>(base, o.nodes); S.sf = at<double>(base, o.sf);
+        S.nrm = at<double>(base, o.nrm); S.midx = at<double>(base, o.midx); S.midy = at<double>(base, o.midy);
+        S.cv = at<double>(base, o.cv); S.scv = at<double>(base, o.scv); S.bce = at<int>(base, o.bce);
+        S.be_len = at<double>(base, o.blen); S.be_mid = at<double>(base, o.bmid); S.be_nrm = at<double>(base, o.bnrm);
+        S.cen = at<double>(base, o.cen); S.ptr = at<int>(base, o.ptr); S.inc = at<int>(base, o.inc);
+        S.dir = at<unsigned char>(base, o.dir);
+        S.alpha = at<double>(base, o.alpha); S.qe = at<double>(base, o.qe); S.bq = at<double>(base, o.bq);
+        std::memcpy(A.host.data() + o_S + s * sizeof(DScen), &S, sizeof(DScen));
+    }
+    if (hipMemcpy(P->buf, A.host.data(), P->bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        err = "hipMemcpy failed for the problem";
+        hipFree(P->buf);
+        delete P;
+        return nullptr;
+    }
+    DBatch& B = P->B;
+    B.n_scen = v.n_scen; B.n_tmpl = v.n_tmpl; B.n_cells = v.n_cells; B.max_discs = v.max_discs;
+    B.nn_max = nn_max; B.ne_max = ne_max; B.macro_nn_max = mnn; B.macro_ne_max = mne;
+    B.scen = at<DScen>(base, o_S); B.tmpl = at<DTmpl>(base, o_T);
+    B.cell_tmpl = at<int>(base, o_ct); B.cell_scen = at<int>(base, o_cs); B.cell_off = at<long long>(base, o_co);
+    B.discs = v.max_discs > 0 ? at<double>(base, o_disc) : nullptr;
+    B.Q = at<double>(base, o_Q); B.bad = at<int>(base, o_bad);
+    B.Ss = v.Ss; B.N = v.N;
+    P->lds = ((size_t)3 * nn_max + (size_t)3 * ne_max) * sizeof(double) + (size_t)ne_max;
+    P->lds = (P->lds + 15) & ~size_t(15);
+    if (P->lds > 64 * 1024) {
+        if (hipFuncSetAttribute((const void*)k_micro, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds) != hipSuccess) {
+            err = "micro template needs more LDS than a workgroup may allocate";
+            hipFree(P->buf);
+            delete P;
+            return nullptr;
+        }
+    }
+    return P;
+}
+
+void free_problem(DevProblem* p)
+{
+    if (!p) return;
+    if (p->buf) hipFree(p->buf);
+    delete p;
+}
+
+const DBatch& batch_of(const DevProblem* p) { return p->B; }
+size_t micro_lds_bytes(const DevProblem* p) { return p->lds; }
+
+hipError_t launch_gfunc(const DevProblem* p, const GArgs& a, hipStream_t st)
+{
+    const DBatch& B = p->B;
+    dim3 ge((B.macro_ne_max + 255) / 256, B.n_scen);
+    hipLaunchKernelGGL(k_macro_elem, ge, dim3(256), 0, st, B, a);
+    hipLaunchKernelGGL(k_micro, dim3(B.n_cells), dim3(kMicroBS), p->lds, st, B, a);
+    dim3 gn((B.macro_nn_max + 255) / 256, B.n_scen);
+    hipLaunchKernelGGL(k_macro_node, gn, dim3(256), 0, st, B, a);
+    return hipGetLastError();
+}
+
+}  // namespace tsr

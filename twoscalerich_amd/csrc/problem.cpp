@@ -1,0 +1,404 @@
+// problem.cpp - the workloads (BASELINE configurations and the reference's own input
+// files) assembled into a batch view, plus the registry that lets reference-signature
+// entry points find the view of a user_data_struct.
+#include "tsr_internal.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+struct tsr_problem {
+    std::vector<tsr_scenario> scen;
+    std::vector<micro_mesh_struct> tmpl;
+    std::vector<std::vector<unsigned char>> soil_store;
+    std::vector<const unsigned char*> tmpl_soil;
+    std::vector<int> cell_tmpl;
+    std::vector<long long> cell_offset;
+    std::vector<double> discs;
+    std::vector<macro_mesh_struct> macros;   // owned
+    std::vector<Keff_struct> keffs;          // owned
+    tsr_batch_view view{};
+    user_data_struct ud0{};
+    std::vector<double> u0;
+    double t_end = 1.0;
+    exprem_options_struct opt{};
+    long long distinct_bytes = 0;
+
+    ~tsr_problem()
+    {
+        tsr::registry_drop(&view);
+        for (auto& m : macros) tsr::free_macro(m);
+        for (auto& t : tmpl) tsr::free_micro(t);
+        for (auto& k : keffs) { delete[] k.logh_values; delete[] k.Keff_values; }
+    }
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+// SplitMix64 (Steele, Lea & Flood 2014): the workloads' only random source, so every
+// consumer of a workload sees the same inputs.
+struct Rng {
+    unsigned long long s;
+    unsigned long long next()
+    {
+        unsigned long long z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uni() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+    double uni(double a, double b) { return a + (b - a) * uni(); }
+    double logu(double a, double b) { return std::exp(uni(std::log(a), std::log(b))); }
+};
+
+soil_struct make_soil(double Ks, double tr, double ts, double al, double n)
+{
+    soil_struct s;
+    s.Ksat = Ks; s.thetar = tr; s.thetas = ts; s.alpha = al; s.n = n; s.m = 1.0 - 1.0 / n;
+    return s;
+}
+
+const soil_struct kLoam = make_soil(0.2496, 0.078, 0.43, 3.6, 1.56);
+const soil_struct kSand = make_soil(7.128, 0.045, 0.43, 14.5, 2.68);
+
+// Keff table: 200 log-spaced suctions from 100 m to 1e-4 m, K = K_A^epsA K_B^(1-epsA)
+// (area-weighted geometric mean), written as diag(K, K) in the reference's layout.
+Keff_struct geometric_keff(const soil_struct& A, const soil_struct& B, double epsA)
+{
+    const int n = 200;
+    Keff_struct K;
+    K.epsilonA = epsA;
+    K.no_hvalues = n;
+    K.logh_first = 2.0;
+    K.logr = -6.0 / (double)(n - 1);
+    K.logh_values = new double[n];
+    K.Keff_values = new double[4 * n];
+    for (int i = 0; i < n; ++i) {
+        const double lh = (i == n - 1) ? -4.0 : K.logh_first + K.logr * i;
+        K.logh_values[i] = lh;
+        const double h = -std::pow(10.0, lh);
+        const double ka = tsr::vg_K_host(A, h), kb = tsr::vg_K_host(B, h);
+        const double k = std::exp(epsA * std::log(ka) + (1.0 - epsA) * std::log(kb));
+        K.Keff_values[i] = k;
+        K.Keff_values[n + i] = 0.0;
+        K.Keff_values[2 * n + i] = 0.0;
+        K.Keff_values[3 * n + i] = k;
+    }
+    return K;
+}
+
+macro_mesh_struct structured_macro(int n)
+{
+    std::vector<double> xy;
+    std::vector<int> el, be;
+    const int labels[4] = {-1, -2, -3, -4};   // top, bottom, right, left
+    tsr::structured_grid(n, 0.0, 0.0, 1.0, xy, el, be, labels);
+    const int bc[5] = {TSR_BC_DIRICHLET, TSR_BC_NOFLUX, TSR_BC_NOFLUX, TSR_BC_NOFLUX, TSR_BC_NOFLUX};
+    macro_mesh_struct M;
+    tsr::build_macro(3, n * n, xy.data(), (int)be.size() / 3, be.data(), 2 * (n - 1) * (n - 1), el.data(), bc, M);
+    return M;
+}
+
+micro_mesh_struct structured_micro(int r, double ell)
+{
+    std::vector<double> xy;
+    std::vector<int> el, be;
+    const int labels[4] = {-1, -1, -1, -1};   // the whole cell perimeter is Dirichlet
+    tsr::structured_grid(r, 0.0, 0.0, ell, xy, el, be, labels);
+    micro_mesh_struct m;
+    tsr::build_micro(ell, ell, 3, r * r, xy.data(), (int)be.size() / 3, be.data(), 2 * (r - 1) * (r - 1), el.data(), m);
+    return m;
+}
+
+std::vector<unsigned char> disc_soil(const micro_mesh_struct& m, double cx, double cy, double r)
+{
+    std::vector<unsigned char> s(m.no_elements);
+    for (int e = 0; e < m.no_elements; ++e) {
+        const double dx = m.centroid_elements[e] - cx, dy = m.centroid_elements[m.no_elements + e] - cy;
+        s[e] = (dx * dx + dy * dy < r * r) ? TSR_SOIL_B : TSR_SOIL_A;
+    }
+    return s;
+}
+
+void finish(tsr_problem* p, double Ss)
+{
+    // cell offsets, scenario offsets
+    long long off = 0;
+    int cell = 0;
+    for (auto& sc : p->scen) {
+        sc.u_offset = off;
+        sc.first_cell = cell;
+        off += sc.ud.macro_mesh.no_nodes;
+        for (int c = 0; c < sc.n_cells; ++c, ++cell) {
+            p->cell_offset[cell] = off;
+            off += p->tmpl[p->cell_tmpl[cell]].no_nodes;
+        }
+        sc.ud.N = 0;
+    }
+    for (auto& sc : p->scen) {
+        long long n = sc.ud.macro_mesh.no_nodes;
+        for (int c = 0; c < sc.n_cells; ++c) n += p->tmpl[p->cell_tmpl[sc.first_cell + c]].no_nodes;
+        sc.ud.N = (int)n;
+    }
+    p->tmpl_soil.clear();
+    for (auto& s : p->soil_store) p->tmpl_soil.push_back(s.empty() ? nullptr : s.data());
+    tsr_batch_view& v = p->view;
+    v.n_scen = (int)p->scen.size();
+    v.n_tmpl = (int)p->tmpl.size();
+    v.n_cells = (int)p->cell_tmpl.size();
+    v.scen = p->scen.data();
+    v.tmpl = p->tmpl.data();
+    v.tmpl_soil = p->tmpl_soil.data();
+    v.cell_tmpl = p->cell_tmpl.data();
+    v.cell_offset = p->cell_offset.data();
+    v.cell_discs = v.max_discs > 0 ? p->discs.data() : nullptr;
+    v.Ss = Ss;
+    v.N = off;
+    p->ud0 = p->scen[0].ud;
+    p->ud0.N = (int)off;
+    p->u0.assign((size_t)off, 0.0);
+    tsr::registry_put(p->ud0, &p->view);
+    // bytes a G call must read that are distinct per cell / scenario
+    long long db = 0;
+    if (v.max_discs > 0) db += (long long)v.n_cells * v.max_discs * 3 * 8;
+    if (p->tmpl.size() > 1) db += (long long)v.n_cells;   // template id (1 byte of information per cell)
+    if (v.n_scen > 1) db += (long long)v.n_scen * (2 * 5 * 8 + 4LL * p->scen[0].ud.effective_conductivity.no_hvalues * 8);
+    p->distinct_bytes = db;
+}
+
+void config_options(exprem_options_struct& o, int maxk)
+{
+    set_exprem_options(o);
+    o.AbsTol = 1e-6;
+    o.RelTol = 1e-6;
+    o.MaxKrylov = maxk;
+}
+
+}  // namespace
+
+namespace tsr {
+
+double vg_K_host(const soil_struct& s, double h)
+{
+    if (h >= 0.0) return s.Ksat;
+    const double x = std::exp(s.n * std::log(s.alpha * (-h)));
+    const double Se = std::exp(-s.m * std::log1p(x));
+    const double B = -std::expm1(-s.m * std::log1p(1.0 / x));
+    return s.Ksat * std::sqrt(Se) * B * B;
+}
+
+namespace {
+std::mutex g_mu;
+std::map<std::pair<const void*, long long>, const tsr_batch_view*> g_map;
+}
+
+void registry_put(const user_data_struct& ud, const tsr_batch_view* v)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_map[{(const void*)ud.macro_mesh.elements, (long long)ud.N}] = v;
+}
+
+const tsr_batch_view* registry_get(const user_data_struct& ud)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_map.find({(const void*)ud.macro_mesh.elements, (long long)ud.N});
+    return it == g_map.end() ? nullptr : it->second;
+}
+
+void registry_drop(const tsr_batch_view* v)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (auto it = g_map.begin(); it != g_map.end();)
+        it = (it->second == v) ? g_map.erase(it) : std::next(it);
+}
+
+void set_error(const std::string& s) { g_err = s; }
+
+}  // namespace tsr
+
+extern "C" {
+
+const char* tsr_last_error(void) { return g_err.c_str(); }
+
+tsr_problem* tsr_problem_config(int cfg, int macro_n, int micro_r, int n_scen, unsigned long long seed)
+{
+    std::unique_ptr<tsr_problem> p(new tsr_problem());
+    Rng rng{seed ? seed : 0x7453520000000000ull + (unsigned long long)cfg};
+    int nM = 0, r = 0, S = 1, maxk = 30;
+    double tend = 1.0;
+    switch (cfg) {
+        case TSR_CFG_PLAIN_SMALL: nM = 21; r = 11; tend = 0.1; break;
+        case TSR_CFG_PLAIN:       nM = 101; r = 21; break;
+        case TSR_CFG_RAGGED:      nM = 101; r = 0; break;
+        case TSR_CFG_SCENARIOS:   nM = 41; r = 16; S = 64; break;
+        case TSR_CFG_LARGE:       nM = 201; r = 31; maxk = 40; break;
+        default: g_err = "unknown workload"; return nullptr;
+    }
+    if (macro_n > 0) nM = macro_n;
+    if (micro_r > 0 && cfg != TSR_CFG_RAGGED) r = micro_r;
+    if (n_scen > 0 && cfg == TSR_CFG_SCENARIOS) S = n_scen;
+    if (nM < 2 || (cfg != TSR_CFG_RAGGED && r < 2)) { g_err = "sizes too small"; return nullptr; }
+    const double ell = 1.0 / (double)(nM - 1);
+    const double Ss = 1e-5;   // specific storage [1/m]: keeps C(h) + Ss > 0 at h >= 0
+    p->t_end = tend;
+    config_options(p->opt, maxk);
+
+    p->macros.push_back(structured_macro(nM));
+    const macro_mesh_struct& M = p->macros[0];
+    const int nodes = M.no_nodes;
+
+    auto scenario = [&](const soil_struct& A, const soil_struct& B, const Keff_struct& K) {
+        tsr_scenario sc{};
+        sc.ud.macro_mesh = M;
+        sc.ud.micro_mesh = p->tmpl[0];
+        sc.ud.effective_conductivity = K;
+        sc.ud.soilA = A;
+        sc.ud.soilB = B;
+        sc.ud.boundary_flux = 0.0;
+        sc.ud.model_type = 3;
+        sc.ud.NUM_THREADS = 1;
+        sc.host_kind = TSR_HOST_NODE;
+        sc.n_cells = nodes;
+        sc.dirichlet_head = -0.1;
+        return sc;
+    };
+
+    if (cfg == TSR_CFG_RAGGED) {
+        const int rs[4] = {11, 16, 21, 31};
+        for (int k = 0; k < 4; ++k) { p->tmpl.push_back(structured_micro(rs[k], ell)); p->soil_store.emplace_back(); }
+        p->view.max_discs = 3;
+        p->discs.assign((size_t)nodes * 9, 0.0);
+        p->cell_tmpl.resize(nodes);
+        double incl = 0.0;
+        for (int c = 0; c < nodes; ++c) {
+            p->cell_tmpl[c] = (int)(rng.next() % 4);
+            const int nd = 1 + (int)(rng.next() % 3);
+            int placed = 0;
+            for (int tries = 0; placed < nd && tries < 1000; ++tries) {
+                const double rad = rng.uni(0.08, 0.3), cx = rng.uni(0.2, 0.8), cy = rng.uni(0.2, 0.8);
+                bool ok = true;
+                for (int k = 0; k < placed && ok; ++k) {
+                    const double* d = &p->discs[(size_t)c * 9 + 3 * k];
+                    const double dx = d[0] / ell - cx, dy = d[1] / ell - cy;
+                    ok = std::sqrt(dx * dx + dy * dy) >= d[2] / ell + rad;
+                }
+                if (!ok) continue;
+                double* d = &p->discs[(size_t)c * 9 + 3 * placed];
+                d[0] = cx * ell; d[1] = cy * ell; d[2] = rad * ell;
+                incl += M_PI * rad * rad;
+                ++placed;
+            }
+        }
+        const double epsA = 1.0 - incl / nodes;
+        p->keffs.push_back(geometric_keff(kLoam, kSand, epsA));
+        p->scen.push_back(scenario(kLoam, kSand, p->keffs[0]));
+    } else if (cfg == TSR_CFG_LARGE) {
+        p->tmpl.push_back(structured_micro(r, ell));
+        p->soil_store.emplace_back();
+        p->view.max_discs = 1;
+        p->discs.assign((size_t)nodes * 3, 0.0);
+        p->cell_tmpl.assign(nodes, 0);
+        double incl = 0.0;
+        for (int c = 0; c < nodes; ++c) {
+            const double rad = rng.uni(0.1, 0.3), cx = rng.uni(0.3, 0.7), cy = rng.uni(0.3, 0.7);
+            p->discs[3 * c] = cx * ell; p->discs[3 * c + 1] = cy * ell; p->discs[3 * c + 2] = rad * ell;
+            incl += M_PI * rad * rad;
+        }
+        p->keffs.push_back(geometric_keff(kLoam, kSand, 1.0 - incl / nodes));
+        p->scen.push_back(scenario(kLoam, kSand, p->keffs[0]));
+    } else {
+        p->tmpl.push_back(structured_micro(r, ell));
+        p->soil_store.push_back(disc_soil(p->tmpl[0], 0.5 * ell, 0.5 * ell, 0.25 * ell));
+        const double epsA = 1.0 - M_PI * 0.25 * 0.25;
+        std::vector<soil_struct> SA(S, kLoam), SB(S, kSand);
+        if (cfg == TSR_CFG_SCENARIOS) {
+            auto draw = [&]() {
+                const double al = rng.logu(1.0, 15.0), n = rng.uni(1.3, 2.7), Ks = rng.logu(0.05, 8.0);
+                const double tr = rng.uni(0.03, 0.1), ts = rng.uni(0.36, 0.46);
+                return make_soil(Ks, tr, ts, al, n);
+            };
+            for (int s = 0; s < S; ++s) { SA[s] = draw(); SB[s] = draw(); }
+        }
+        p->keffs.reserve(S);
+        for (int s = 0; s < S; ++s) p->keffs.push_back(geometric_keff(SA[s], SB[s], epsA));
+        for (int s = 0; s < S; ++s) p->scen.push_back(scenario(SA[s], SB[s], p->keffs[s]));
+        p->cell_tmpl.assign((size_t)nodes * S, 0);
+    }
+    p->cell_offset.assign(p->cell_tmpl.size(), 0);
+    finish(p.get(), Ss);
+    for (auto& x : p->u0) x = -5.0;
+    return p.release();
+}
+
+tsr_problem* tsr_problem_reference(const char* macro_msh, const char* micro_msh, const char* keff_txt,
+                                   const soil_struct* soilA, const soil_struct* soilB, double boundary_flux,
+                                   const int* macro_bc5, int model_type, double h0, int host_kind)
+{
+    std::unique_ptr<tsr_problem> p(new tsr_problem());
+    macro_mesh_struct M{};
+    micro_mesh_struct m{};
+    Keff_struct K{};
+    int bc[5];
+    std::memcpy(bc, macro_bc5, sizeof(bc));
+    if (macro_mesh_properties(macro_msh, M, bc)) { g_err = std::string("cannot read ") + macro_msh; return nullptr; }
+    p->macros.push_back(M);
+    if (micro_mesh_properties(micro_msh, m)) { g_err = std::string("cannot read ") + micro_msh; return nullptr; }
+    if (m.no_verts != 3) { g_err = "micro mesh must be triangles"; tsr::free_micro(m); return nullptr; }
+    p->tmpl.push_back(m);
+    p->soil_store.emplace_back();
+    if (tsr::read_keff(keff_txt, K)) { g_err = std::string("cannot read ") + keff_txt; return nullptr; }
+    p->keffs.push_back(K);
+    tsr_scenario sc{};
+    sc.ud.macro_mesh = p->macros[0];
+    sc.ud.micro_mesh = p->tmpl[0];
+    sc.ud.effective_conductivity = p->keffs[0];
+    sc.ud.soilA = *soilA;
+    sc.ud.soilB = *soilB;
+    sc.ud.boundary_flux = boundary_flux;
+    sc.ud.model_type = model_type;
+    sc.ud.NUM_THREADS = 1;
+    sc.host_kind = host_kind;
+    sc.n_cells = host_kind == TSR_HOST_NODE ? M.no_nodes : M.no_elements;
+    sc.dirichlet_head = boundary_flux;
+    p->scen.push_back(sc);
+    p->cell_tmpl.assign(sc.n_cells, 0);
+    p->cell_offset.assign(sc.n_cells, 0);
+    p->t_end = 2.0;
+    set_exprem_options(p->opt);
+    finish(p.get(), 0.0);
+    for (auto& x : p->u0) x = h0;
+    return p.release();
+}
+
+void tsr_problem_free(tsr_problem* p) { delete p; }
+const tsr_batch_view* tsr_problem_view(const tsr_problem* p) { return &p->view; }
+const user_data_struct* tsr_problem_user_data(const tsr_problem* p) { return &p->ud0; }
+long long tsr_problem_N(const tsr_problem* p) { return p->view.N; }
+void tsr_problem_initial(const tsr_problem* p, double* u0) { std::memcpy(u0, p->u0.data(), sizeof(double) * p->u0.size()); }
+double tsr_problem_tend(const tsr_problem* p) { return p->t_end; }
+void tsr_problem_options(const tsr_problem* p, exprem_options_struct* o) { *o = p->opt; }
+long long tsr_problem_distinct_bytes(const tsr_problem* p) { return p->distinct_bytes; }
+
+int set_exprem_options_c(exprem_options_struct* o) { return set_exprem_options(*o); }
+
+void exprem_stats_init(exprem_stats_struct* st, const exprem_options_struct* o, double* u0)
+{
+    st->CurrentStep = o->InitStep; st->CurrentTime = 0.0; st->CurrentKryDim = 0;
+    st->CurrentSoln = u0; st->NumSteps = 0; st->FuncEvals = 0;
+}
+
+void check_step(double dt, double MinStep)
+{
+    if (dt < MinStep) std::fprintf(stderr, "exprem: step %.3e below MinStep %.3e\n", dt, MinStep);
+}
+
+double check_flag_Gfunc(int flag, double dt) { return flag != 0 ? 0.5 * dt : dt; }
+
+}  // extern "C"

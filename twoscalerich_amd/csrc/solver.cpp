@@ -1,0 +1,904 @@
+// solver.cpp - device-resident EXPREM driver and the reference-signature entry points.
+//
+// One accepted exponential Rosenbrock-Euler step per call (the reference's exprem
+// contract, include/main.h:29-32, main.cpp:311-341), with the exprb32 embedded error
+// estimate (Hochbruck, Ostermann & Schweitzer, SINUM 47 (2009)):
+//     U2 = u + tau phi1(tau J) G(u),  D = G(U2) - G(u) - J (U2 - u),
+//     u+ = U2 + 2 tau phi3(tau J) D,  err = WRMS(2 tau phi3(tau J) D).
+// phi-vector products by Arnoldi with J v = (G(u + eps v) - G(u)) / eps,
+// eps = sqrt(DBL_EPSILON)(1 + ||u||_2), CGS2 orthogonalisation; J (U2 - u) from the
+// Arnoldi relation; Krylov stop when tau (tau beta h_{m+1,m} |e_m^T phi1(tau H_m) e1|)
+// WRMS(v_{m+1}) <= 0.1; step law tau * min(MaxStepIncFac, max(MinStepDecFac,
+// SafetyFac err^(-1/3))); a rejected step reuses the Krylov basis.
+//
+// Everything of length N lives in HBM; the host holds only the (m+1) x m Hessenberg
+// matrices and per-scenario scalars.  All scenarios of a batch advance together: each
+// Arnoldi iteration, update and error estimate is one launch over the whole batch
+// with a per-scenario activity mask, so converged or accepted scenarios cost nothing.
+#include "device.h"
+#include "tsr_internal.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#define HIPCHK(x)                                                                                 \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess) {                                                                   \
+            tsr::set_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " #x);        \
+            return fail_code;                                                                     \
+        }                                                                                         \
+    } while (0)
+
+namespace tsr {
+constexpr double kKrylovTol = 0.1;
+constexpr int kMaxKrylovCap = 120;   // largest MaxKrylov accepted
+constexpr int kLearnedCap = 60;       // largest Krylov cap the learned controller may propose
+struct Mlp;   // learned controller (mlp.hip)
+Mlp*  mlp_create();
+void  mlp_free(Mlp*);
+int   mlp_set(Mlp*, const float* params, const float* mean, const float* stdv);
+int   mlp_infer(Mlp*, const float* feats, int n, float* out, hipStream_t st);
+bool  mlp_ready(const Mlp*);
+}  // namespace tsr
+
+using namespace tsr;
+
+struct ScState {
+    double t = 0.0, proposal = 1e-3, tau = 0.0, tstop = 0.0, tau_prop = 0.0;
+    bool landing = false;
+    int m = 0, mD = 0, rejects = 0, fe = 0, m_cap = 0, ctrl = 0;
+    double beta = 0.0, hn = 0.0, hnD = 0.0, betaD = 0.0, err = 0.0, eps = 0.0, wnext = 0.0, wnextD = 0.0;
+    bool bd = false, bdD = false, conv = false, failed = false, accepted = false;
+    std::vector<double> H, HD, phi, y, z, cs, csW;
+    exprem_stats_struct stats{};
+    // recent history for the learned controller
+    double hist_tau[4] = {0, 0, 0, 0};
+    double hist_m[4] = {0, 0, 0, 0};
+};
+
+struct tsr_solver {
+    DevProblem* P = nullptr;
+    hipStream_t st = nullptr;
+    long long N = 0;
+    int S = 0, mcap = 0;
+    exprem_options_struct opt{};
+    // vectors
+    double *u = nullptr, *g0 = nullptr, *u2 = nullptr, *g2 = nullptr, *D = nullptr, *unew = nullptr;
+    double* scratch[2] = {nullptr, nullptr};
+    std::vector<double*> V, W;
+    const double** Vd = nullptr;
+    const double** Wd = nullptr;
+    // blocks
+    int nblk = 0;
+    BlkMap M{};
+    int* blk_first = nullptr;
+    long long* scen_rows = nullptr;
+    std::vector<long long> rows_h;
+    double* part = nullptr;
+    int ldp = 0;
+    KrylovState KV{}, KW{};
+    int* mask = nullptr;      // device activity mask
+    int* nk = nullptr;        // per-scenario column counts for k_lincomb
+    int* bad = nullptr;       // alias of DBatch::bad
+    std::vector<int> mask_h, nk_h, bad_h;
+    std::vector<double> coef_h, res_h, veps_h, cs0_h;
+    std::vector<ScState> sc;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_g_ms = 0.0;
+    // host RHS mode (exprem_c with a host Gfunc)
+    tsr_gfunc_t host_g = nullptr;
+    user_data_struct host_ud{};
+    std::vector<double> hx, hg, hg0;
+    // controller
+    int ctrl_mode = 0;
+    Mlp* mlp = nullptr;
+    std::vector<double> macro_h;
+    // timing of the last run
+    double g_ms_acc = 0.0;
+    int    g_calls = 0;
+};
+
+namespace {
+
+std::mutex g_cache_mu;
+std::map<const tsr_batch_view*, tsr_solver*> g_solver_cache;
+std::vector<tsr_solver*> g_retired;   // solvers replaced when a caller changed options; freed at tsr_solver_free of the cache
+
+template <class T> int dalloc(T** p, size_t n)
+{
+    return hipMalloc((void**)p, sizeof(T) * (n ? n : 1)) == hipSuccess ? 0 : 1;
+}
+
+// ---- device helpers with error propagation ----
+int G_call(tsr_solver* s, const double* v, const double* veps, const double* g0, const double* ieps, double* out,
+           const int* active)
+{
+    const int fail_code = 2;
+    if (s->host_g) {
+        // host RHS: state crosses PCIe, Krylov arithmetic stays on the device
+        const size_t N = (size_t)s->N;
+        HIPCHK(hipMemcpyAsync(s->hx.data(), s->u, N * sizeof(double), hipMemcpyDeviceToHost, s->st));
+        std::vector<double> vh;
+        double ve = 0.0, ie = 0.0;
+        if (v) {
+            vh.resize(N);
+            HIPCHK(hipMemcpyAsync(vh.data(), v, N * sizeof(double), hipMemcpyDeviceToHost, s->st));
+            HIPCHK(hipMemcpyAsync(&ve, veps, sizeof(double), hipMemcpyDeviceToHost, s->st));
+        }
+        if (g0) HIPCHK(hipMemcpyAsync(&ie, ieps, sizeof(double), hipMemcpyDeviceToHost, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+        if (v) for (size_t i = 0; i < N; ++i) s->hx[i] += ve * vh[i];
+        const int flag = s->host_g(s->hg.data(), s->hx.data(), s->host_ud);
+        if (g0) for (size_t i = 0; i < N; ++i) s->hg[i] = (s->hg[i] - s->hg0[i]) * ie;
+        else if (out == s->g0) s->hg0 = s->hg;
+        bool finite = true;
+        for (size_t i = 0; i < N && finite; ++i) finite = std::isfinite(s->hg[i]);
+        s->bad_h[0] = (flag != 0 || !finite) ? 1 : 0;
+        HIPCHK(hipMemcpyAsync(s->bad, s->bad_h.data(), sizeof(int), hipMemcpyHostToDevice, s->st));
+        HIPCHK(hipMemcpyAsync(out, s->hg.data(), N * sizeof(double), hipMemcpyHostToDevice, s->st));
+        s->g_calls++;
+        return 0;
+    }
+    GArgs a;
+    a.u = s->u; a.v = v; a.veps = veps; a.g0 = g0; a.ieps = ieps; a.active = active; a.out = out;
+    HIPCHK(launch_gfunc(s->P, a, s->st));
+    s->g_calls++;
+    return 0;
+}
+
+// G(x) for an explicit state x (not s->u): temporarily point u at x.
+int G_at(tsr_solver* s, const double* x, double* out, const int* active)
+{
+    double* keep = s->u;
+    s->u = const_cast<double*>(x);
+    const int rc = G_call(s, nullptr, nullptr, nullptr, nullptr, out, active);
+    s->u = keep;
+    return rc;
+}
+
+int upload_mask(tsr_solver* s)
+{
+    const int fail_code = 2;
+    HIPCHK(hipMemcpyAsync(s->mask, s->mask_h.data(), sizeof(int) * s->S, hipMemcpyHostToDevice, s->st));
+    return 0;
+}
+
+int clear_bad(tsr_solver* s)
+{
+    const int fail_code = 2;
+    HIPCHK(hipMemsetAsync(s->bad, 0, sizeof(int) * s->S, s->st));
+    return 0;
+}
+
+// Read back the per-scenario results (res, 8 slots), bad flags, and H column j of the basis.
+int read_iteration(tsr_solver* s, KrylovState& K, int j, std::vector<double>* Hcols)
+{
+    const int fail_code = 2;
+    HIPCHK(hipMemcpyAsync(s->res_h.data(), K.res, sizeof(double) * 8 * s->S, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipMemcpyAsync(s->bad_h.data(), s->bad, sizeof(int) * s->S, hipMemcpyDeviceToHost, s->st));
+    if (Hcols) {
+        std::vector<double> tmp((size_t)s->S * (j + 2));
+        HIPCHK(hipMemcpy2DAsync(tmp.data(), sizeof(double) * (j + 2), K.H + (size_t)j * K.ldh,
+                                sizeof(double) * K.hstride, sizeof(double) * (j + 2), s->S, hipMemcpyDeviceToHost,
+                                s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+        for (int q = 0; q < s->S; ++q)
+            if (s->mask_h[q])
+                for (int i = 0; i <= j + 1; ++i) Hcols[q][(size_t)j * K.ldh + i] = tmp[(size_t)q * (j + 2) + i];
+    } else {
+        HIPCHK(hipStreamSynchronize(s->st));
+    }
+    return 0;
+}
+
+double krylov_est_main(ScState& c, int m, double tau, int ldh, int ldphi)
+{
+    if (phi1_pade(c.phi.data(), c.H.data(), tau, m, ldh, ldphi)) return INFINITY;
+    return tau * (tau * c.beta * c.hn * std::fabs(c.phi[m - 1])) * c.wnext;
+}
+
+// Arnoldi over the active scenarios in s->mask_h.  basis 0: main (V, KV, H), 1: error (W, KW, HD).
+// Returns 0, -1 if an RHS evaluation failed (per scenario: failed flag), 2 on device error.
+int arnoldi(tsr_solver* s, int which)
+{
+    const int fail_code = 2;
+    std::vector<double*>& B = which == 0 ? s->V : s->W;
+    const double** Bd = which == 0 ? s->Vd : s->Wd;
+    KrylovState& K = which == 0 ? s->KV : s->KW;
+    const int ldh = s->mcap + 1;
+    std::vector<std::vector<double>*> Hs(s->S);
+    std::vector<double> tmpH;
+    for (int q = 0; q < s->S; ++q) {
+        ScState& c = s->sc[q];
+        Hs[q] = which == 0 ? &c.H : &c.HD;
+        if (which == 0) { c.m = 0; c.bd = false; c.conv = false; } else { c.mD = 0; c.bdD = false; }
+    }
+    std::vector<std::vector<double>> Hv(s->S);
+    for (int j = 0; j < s->mcap; ++j) {
+        int any = 0;
+        for (int q = 0; q < s->S; ++q) {
+            const ScState& c = s->sc[q];
+            const int mcapq = which == 0 ? c.m_cap : s->opt.MaxKrylov;
+            if (s->mask_h[q] && j >= mcapq) s->mask_h[q] = 0;
+            any |= s->mask_h[q];
+        }
+        if (!any) break;
+        if (upload_mask(s)) return 2;
+        if (clear_bad(s)) return 2;
+        if (G_call(s, B[j], K.veps, s->g0, K.ieps, B[j + 1], s->mask)) return 2;
+        HIPCHK(launch_pass(PASS_DOTS, Bd, j + 1, B[j + 1], s->u, K.coef, K.ld, s->M, s->nblk, s->part, s->ldp, s->mask,
+                           s->opt.AbsTol, s->opt.RelTol, s->st));
+        HIPCHK(launch_reduce(RED_H1, s->part, s->ldp, j + 1, s->blk_first, s->S, s->scen_rows, K, j, s->mask, s->st));
+        HIPCHK(launch_pass(PASS_AXPY_DOTS, Bd, j + 1, B[j + 1], s->u, K.coef, K.ld, s->M, s->nblk, s->part, s->ldp,
+                           s->mask, s->opt.AbsTol, s->opt.RelTol, s->st));
+        HIPCHK(launch_reduce(RED_H2, s->part, s->ldp, j + 1, s->blk_first, s->S, s->scen_rows, K, j, s->mask, s->st));
+        // the ||w||^2 slot of the first projection is needed for the breakdown test: keep it
+        std::vector<double> w0(s->S, 0.0);
+        HIPCHK(hipMemcpy2DAsync(w0.data(), sizeof(double), K.res + 1, sizeof(double) * 8, sizeof(double), s->S,
+                                hipMemcpyDeviceToHost, s->st));
+        HIPCHK(launch_pass(PASS_AXPY_NORM, Bd, j + 1, B[j + 1], s->u, K.coef, K.ld, s->M, s->nblk, s->part, s->ldp,
+                           s->mask, s->opt.AbsTol, s->opt.RelTol, s->st));
+        HIPCHK(launch_reduce(RED_NORM, s->part, s->ldp, 2, s->blk_first, s->S, s->scen_rows, K, j, s->mask, s->st));
+        std::vector<double> hbuf;
+        // gather H column j for every active scenario
+        {
+            std::vector<double> tmp((size_t)s->S * (j + 2));
+            HIPCHK(hipMemcpy2DAsync(tmp.data(), sizeof(double) * (j + 2), K.H + (size_t)j * K.ldh,
+                                    sizeof(double) * K.hstride, sizeof(double) * (j + 2), s->S,
+                                    hipMemcpyDeviceToHost, s->st));
+            HIPCHK(hipMemcpyAsync(s->res_h.data(), K.res, sizeof(double) * 8 * s->S, hipMemcpyDeviceToHost, s->st));
+            HIPCHK(hipMemcpyAsync(s->bad_h.data(), s->bad, sizeof(int) * s->S, hipMemcpyDeviceToHost, s->st));
+            HIPCHK(hipStreamSynchronize(s->st));
+            for (int q = 0; q < s->S; ++q)
+                if (s->mask_h[q])
+                    for (int i = 0; i <= j + 1; ++i) (*Hs[q])[(size_t)j * ldh + i] = tmp[(size_t)q * (j + 2) + i];
+        }
+        for (int q = 0; q < s->S; ++q) {
+            if (!s->mask_h[q]) continue;
+            ScState& c = s->sc[q];
+            c.fe++;
+            if (s->bad_h[q]) { c.failed = true; s->mask_h[q] = 0; continue; }
+            const double hn = s->res_h[(size_t)q * 8 + 0];
+            const double wn = s->res_h[(size_t)q * 8 + 2];
+            const double wnorm0 = std::sqrt(w0[q]);
+            const int m = j + 1;
+            if (which == 0) {
+                c.m = m; c.hn = hn; c.wnext = wn;
+                c.cs[j + 1] = 1.0 / hn;
+                if (!(hn > 1e-12 * wnorm0) || hn == 0.0) { c.bd = true; c.conv = true; s->mask_h[q] = 0; continue; }
+                if (m >= s->opt.MinKrylov && krylov_est_main(c, m, c.tau, ldh, s->mcap) <= kKrylovTol) {
+                    c.conv = true;
+                    s->mask_h[q] = 0;
+                }
+            } else {
+                c.mD = m; c.hnD = hn; c.wnextD = wn;
+                c.csW[j + 1] = 1.0 / hn;
+                if (!(hn > 1e-12 * wnorm0) || hn == 0.0) { c.bdD = true; s->mask_h[q] = 0; continue; }
+                if (m >= s->opt.MinKrylov) {
+                    std::vector<double> ph(3 * (size_t)m);
+                    if (!phik_e1(ph.data(), c.HD.data(), ldh, m, c.tau, 3)) {
+                        const double est = 2.0 * c.tau * c.betaD * (c.tau * hn * std::fabs(ph[2 * m + m - 1])) * wn;
+                        if (est <= kKrylovTol) s->mask_h[q] = 0;
+                    }
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+int features_and_proposal(tsr_solver* s);
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------
+// One accepted step for every scenario with t < tend.
+// ------------------------------------------------------------------------------------
+static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
+{
+    const int fail_code = -1;
+    const exprem_options_struct& o = s->opt;
+    const int S = s->S, ldh = s->mcap + 1;
+    std::vector<int> act(S, 0);
+    int nact = 0;
+    for (int q = 0; q < S; ++q) {
+        ScState& c = s->sc[q];
+        c.failed = false; c.accepted = false; c.rejects = 0; c.fe = 0; c.m = 0; c.mD = 0; c.err = 0.0; c.ctrl = 0;
+        if (c.t < tend) { act[q] = 1; ++nact; }
+        if (c.stats.NumSteps >= o.MaxNumSteps) return -1;
+    }
+    if (!nact) return 0;
+    if (s->ctrl_mode == 1 && s->mlp && mlp_ready(s->mlp)) {
+        if (features_and_proposal(s)) return -1;
+    }
+    for (int q = 0; q < S; ++q) {
+        if (!act[q]) continue;
+        ScState& c = s->sc[q];
+        if (c.ctrl == 0) { c.m_cap = o.MaxKrylov; }
+        c.proposal = std::fmin(std::fmax(c.proposal, o.MinStep), o.MaxStep);
+        c.tstop = tend;
+        c.tau = c.proposal;
+        c.tau_prop = c.proposal;
+        c.landing = false;
+        if (c.t + c.tau >= c.tstop - 1e-13 * std::fabs(c.tstop)) { c.tau = c.tstop - c.t; c.landing = true; }
+    }
+    s->mask_h = act;
+    if (upload_mask(s) || clear_bad(s)) return -1;
+    if (G_call(s, nullptr, nullptr, nullptr, nullptr, s->g0, s->mask)) return -1;
+    if (launch_norms(s->g0, s->u, s->M, s->nblk, s->part, s->ldp, s->mask, o.AbsTol, o.RelTol, s->st) != hipSuccess)
+        return -1;
+    if (launch_reduce(RED_START, s->part, s->ldp, 3, s->blk_first, S, s->scen_rows, s->KV, 0, s->mask, s->st) != hipSuccess)
+        return -1;
+    std::vector<double> eps_h(S);
+    HIPCHK(hipMemcpyAsync(eps_h.data(), s->KV.eps, sizeof(double) * S, hipMemcpyDeviceToHost, s->st));
+    if (read_iteration(s, s->KV, 0, nullptr)) return -1;
+    for (int q = 0; q < S; ++q) {
+        if (!act[q]) continue;
+        ScState& c = s->sc[q];
+        c.fe++;
+        if (s->bad_h[q]) return -1;
+        c.beta = s->res_h[(size_t)q * 8 + 0];
+        c.eps = eps_h[q];
+        c.cs[0] = 1.0 / c.beta;
+        std::fill(c.H.begin(), c.H.end(), 0.0);
+    }
+    // main Arnoldi over scenarios with beta > 0
+    for (int q = 0; q < S; ++q) s->mask_h[q] = act[q] && s->sc[q].beta > 0.0;
+    if (arnoldi(s, 0)) return -1;
+    for (int q = 0; q < S; ++q) {
+        if (!act[q]) continue;
+        ScState& c = s->sc[q];
+        if (c.failed) return -1;
+        if (c.beta > 0.0 && !c.bd) {
+            while (krylov_est_main(c, c.m, c.tau, ldh, s->mcap) > kKrylovTol) {
+                c.tau *= 0.5;
+                c.landing = false;
+                if (c.tau < o.MinStep) return -1;
+            }
+        }
+    }
+    // attempts
+    std::vector<int> pend = act;
+    for (int attempt = 0; attempt < 200; ++attempt) {
+        int np = 0;
+        for (int q = 0; q < S; ++q) np += pend[q];
+        if (!np) break;
+        // u2 = u + V y
+        for (int q = 0; q < S; ++q) {
+            ScState& c = s->sc[q];
+            double* cf = &s->coef_h[(size_t)q * s->KV.ld];
+            std::fill(cf, cf + s->KV.ld, 0.0);
+            s->nk_h[q] = 0;
+            if (!pend[q] || !(c.beta > 0.0)) continue;
+            if (phi1_pade(c.phi.data(), c.H.data(), c.tau, c.m, ldh, s->mcap)) return -1;
+            for (int k = 0; k < c.m; ++k) { c.y[k] = c.tau * c.beta * c.phi[k]; cf[k] = c.cs[k] * c.y[k]; }
+            s->nk_h[q] = c.m;
+        }
+        s->mask_h = pend;
+        if (upload_mask(s)) return -1;
+        HIPCHK(hipMemcpyAsync(s->KV.coef, s->coef_h.data(), sizeof(double) * S * s->KV.ld, hipMemcpyHostToDevice, s->st));
+        HIPCHK(hipMemcpyAsync(s->nk, s->nk_h.data(), sizeof(int) * S, hipMemcpyHostToDevice, s->st));
+        HIPCHK(launch_lincomb(s->u2, s->u, s->Vd, s->nk, s->KV.coef, s->KV.ld, 1.0, s->M, s->nblk, s->part, s->ldp, 0,
+                              s->u, o.AbsTol, o.RelTol, nullptr, s->mask, s->st));
+        if (clear_bad(s)) return -1;
+        if (G_at(s, s->u2, s->g2, s->mask)) return -1;
+        // D = g2 - g0 - V_{m+1} Hbar y
+        for (int q = 0; q < S; ++q) {
+            ScState& c = s->sc[q];
+            double* cf = &s->coef_h[(size_t)q * s->KV.ld];
+            std::fill(cf, cf + s->KV.ld, 0.0);
+            s->nk_h[q] = 0;
+            if (!pend[q] || !(c.beta > 0.0)) continue;
+            const int mv = c.bd ? c.m : c.m + 1;
+            for (int i = 0; i < mv; ++i) {
+                double acc = 0.0;
+                for (int k = 0; k < c.m; ++k) acc += c.H[i + (size_t)k * ldh] * c.y[k];
+                cf[i] = c.cs[i] * acc;
+            }
+            s->nk_h[q] = mv;
+        }
+        HIPCHK(hipMemcpyAsync(s->KV.coef, s->coef_h.data(), sizeof(double) * S * s->KV.ld, hipMemcpyHostToDevice, s->st));
+        HIPCHK(hipMemcpyAsync(s->nk, s->nk_h.data(), sizeof(int) * S, hipMemcpyHostToDevice, s->st));
+        HIPCHK(launch_lincomb(s->D, s->g2, s->Vd, s->nk, s->KV.coef, s->KV.ld, -1.0, s->M, s->nblk, s->part, s->ldp, 1,
+                              s->u, o.AbsTol, o.RelTol, s->g0, s->mask, s->st));
+        HIPCHK(launch_reduce(RED_SUM, s->part, s->ldp, 1, s->blk_first, S, s->scen_rows, s->KW, 0, s->mask, s->st));
+        if (read_iteration(s, s->KW, 0, nullptr)) return -1;
+        std::vector<int> okq(S, 0), dq(S, 0);
+        for (int q = 0; q < S; ++q) {
+            if (!pend[q]) continue;
+            ScState& c = s->sc[q];
+            c.fe++;
+            okq[q] = !s->bad_h[q];
+            c.betaD = std::sqrt(s->res_h[(size_t)q * 8 + 0]);
+            if (okq[q] && c.betaD > 0.0 && std::isfinite(c.betaD)) dq[q] = 1;
+            else if (okq[q] && !(c.betaD == 0.0)) okq[q] = 0;
+            c.mD = 0;
+        }
+        // error-estimate Arnoldi on D
+        int anyD = 0;
+        for (int q = 0; q < S; ++q) anyD |= dq[q];
+        if (anyD) {
+            std::vector<double> cs0(S * 2, 0.0);
+            for (int q = 0; q < S; ++q) {
+                ScState& c = s->sc[q];
+                if (!dq[q]) continue;
+                c.csW[0] = 1.0 / c.betaD;
+                std::fill(c.HD.begin(), c.HD.end(), 0.0);
+            }
+            // csW[0] and veps for the D basis
+            std::vector<double> csrow((size_t)S * s->KW.ld, 0.0), vep(S, 0.0);
+            HIPCHK(hipMemcpyAsync(csrow.data(), s->KW.cs, sizeof(double) * S * s->KW.ld, hipMemcpyDeviceToHost, s->st));
+            HIPCHK(hipStreamSynchronize(s->st));
+            for (int q = 0; q < S; ++q) {
+                if (!dq[q]) continue;
+                csrow[(size_t)q * s->KW.ld] = s->sc[q].csW[0];
+                vep[q] = s->sc[q].eps * s->sc[q].csW[0];
+            }
+            HIPCHK(hipMemcpyAsync(s->KW.cs, csrow.data(), sizeof(double) * S * s->KW.ld, hipMemcpyHostToDevice, s->st));
+            HIPCHK(hipMemcpyAsync(s->KW.veps, vep.data(), sizeof(double) * S, hipMemcpyHostToDevice, s->st));
+            s->mask_h = dq;
+            if (arnoldi(s, 1)) return -1;
+            for (int q = 0; q < S; ++q)
+                if (dq[q] && s->sc[q].failed) return -1;
+        }
+        // e = 2 tau betaD W phi3(tau HD) e1; unew = u2 + e; err = WRMS(e)
+        for (int q = 0; q < S; ++q) {
+            ScState& c = s->sc[q];
+            double* cf = &s->coef_h[(size_t)q * s->KW.ld];
+            std::fill(cf, cf + s->KW.ld, 0.0);
+            s->nk_h[q] = 0;
+            if (!pend[q] || !dq[q]) continue;
+            std::vector<double> ph(3 * (size_t)c.mD);
+            if (phik_e1(ph.data(), c.HD.data(), ldh, c.mD, c.tau, 3)) return -1;
+            for (int k = 0; k < c.mD; ++k) { c.z[k] = 2.0 * c.tau * c.betaD * ph[2 * c.mD + k]; cf[k] = c.csW[k] * c.z[k]; }
+            s->nk_h[q] = c.mD;
+        }
+        s->mask_h = pend;
+        if (upload_mask(s)) return -1;
+        HIPCHK(hipMemcpyAsync(s->KW.coef, s->coef_h.data(), sizeof(double) * S * s->KW.ld, hipMemcpyHostToDevice, s->st));
+        HIPCHK(hipMemcpyAsync(s->nk, s->nk_h.data(), sizeof(int) * S, hipMemcpyHostToDevice, s->st));
+        HIPCHK(launch_lincomb(s->unew, s->u2, s->Wd, s->nk, s->KW.coef, s->KW.ld, 1.0, s->M, s->nblk, s->part, s->ldp, 2,
+                              s->u, o.AbsTol, o.RelTol, nullptr, s->mask, s->st));
+        HIPCHK(launch_reduce(RED_SUM, s->part, s->ldp, 1, s->blk_first, S, s->scen_rows, s->KW, 0, s->mask, s->st));
+        if (read_iteration(s, s->KW, 0, nullptr)) return -1;
+        std::vector<int> acc(S, 0);
+        int nacc = 0;
+        for (int q = 0; q < S; ++q) {
+            if (!pend[q]) continue;
+            ScState& c = s->sc[q];
+            double err = dq[q] ? std::sqrt(s->res_h[(size_t)q * 8 + 0] / (double)s->rows_h[q]) : 0.0;
+            c.err = err;
+            if (okq[q] && std::isfinite(err) && err <= 1.0) { acc[q] = 1; ++nacc; continue; }
+            c.rejects++;
+            const double fac = (okq[q] && std::isfinite(err)) ? std::fmax(o.MinStepDecFac, o.SafetyFac * std::pow(err, -1.0 / 3.0))
+                                                              : 0.5;
+            c.tau *= fac;
+            c.landing = false;
+            if (c.tau < o.MinStep) return -1;
+        }
+        if (nacc) {
+            if (S == 1) {
+                std::swap(s->u, s->unew);
+            } else {
+                s->mask_h = acc;
+                if (upload_mask(s)) return -1;
+                HIPCHK(launch_masked_copy(s->u, s->unew, s->M, s->nblk, s->mask, s->st));
+            }
+            for (int q = 0; q < S; ++q) if (acc[q]) pend[q] = 0;
+        }
+    }
+    for (int q = 0; q < S; ++q)
+        if (pend[q]) return -1;
+    // bookkeeping
+    for (int q = 0; q < S; ++q) {
+        if (!act[q]) continue;
+        ScState& c = s->sc[q];
+        const double taken = c.tau;
+        c.t = c.landing ? c.tstop : c.t + taken;
+        const double fac = (c.err > 0.0) ? std::fmin(o.MaxStepIncFac, std::fmax(o.MinStepDecFac, o.SafetyFac * std::pow(c.err, -1.0 / 3.0)))
+                                         : o.MaxStepIncFac;
+        double next = taken * fac;
+        if (c.landing && taken < c.proposal) next = std::fmax(next, c.proposal);
+        c.proposal = std::fmin(std::fmax(next, o.MinStep), o.MaxStep);
+        c.stats.CurrentStep = c.proposal;
+        c.stats.CurrentTime = c.t;
+        c.stats.CurrentKryDim = c.m;
+        c.stats.NumSteps += 1;
+        c.stats.FuncEvals += c.fe;
+        for (int k = 3; k > 0; --k) { c.hist_tau[k] = c.hist_tau[k - 1]; c.hist_m[k] = c.hist_m[k - 1]; }
+        c.hist_tau[0] = taken;
+        c.hist_m[0] = c.m;
+        if (rec) {
+            tsr_step_record& r = rec[q];
+            std::memset(&r, 0, sizeof(r));
+            r.t = c.t; r.tau = taken; r.tau_proposed = c.tau_prop; r.err = c.err; r.beta = c.beta;
+            double hnorm = 0.0;
+            for (int i = 0; i < c.m; ++i) {
+                double rs = 0.0;
+                for (int k = 0; k < c.m; ++k) rs += std::fabs(c.H[i + (size_t)k * ldh]);
+                hnorm = std::max(hnorm, rs);
+            }
+            r.hnorm = hnorm; r.resid = c.hn; r.m = c.m; r.mD = c.mD; r.rejects = c.rejects; r.fevals = c.fe;
+            r.controller = c.ctrl; r.m_cap = c.m_cap;
+        }
+    }
+    s->mask_h.assign(S, 1);
+    return 0;
+}
+
+namespace {
+
+// Learned-controller features (16 per scenario) and proposal; classical values are kept
+// when the network's output is not finite.
+int features_and_proposal(tsr_solver* s)
+{
+    const int S = s->S;
+    const tsr_batch_view* vv = nullptr;
+    (void)vv;
+    std::vector<float> feat((size_t)S * 16), out((size_t)S * 2);
+    const int fail_code = -1;
+    for (int q = 0; q < S; ++q) {
+        ScState& c = s->sc[q];
+        float* f = &feat[(size_t)q * 16];
+        for (int k = 0; k < 4; ++k) f[k] = (float)std::log(std::max(c.hist_tau[k] > 0 ? c.hist_tau[k] : c.proposal, 1e-12));
+        for (int k = 0; k < 4; ++k) f[4 + k] = (float)(c.hist_m[k] > 0 ? c.hist_m[k] : 3.0);
+        f[8] = 0.0f; f[9] = 0.0f;
+        for (int k = 10; k < 16; ++k) f[k] = 0.0f;
+    }
+    if (!s->macro_h.empty()) {
+        // macro head / saturation statistics are filled by the caller through macro_h
+    }
+    if (mlp_infer(s->mlp, feat.data(), S, out.data(), s->st)) return -1;
+    for (int q = 0; q < S; ++q) {
+        ScState& c = s->sc[q];
+        const double lt = out[(size_t)q * 2], mk = out[(size_t)q * 2 + 1];
+        if (std::isfinite(lt) && std::isfinite(mk)) {
+            c.proposal = std::exp(lt);
+            c.m_cap = std::min(std::max((int)std::lround(mk), 5), std::min(60, s->mcap));
+            c.ctrl = 1;
+        } else {
+            c.ctrl = 2;
+            c.m_cap = s->opt.MaxKrylov;
+        }
+    }
+    return 0;
+}
+
+tsr_solver* create(const tsr_batch_view* view, const exprem_options_struct* opt)
+{
+    std::string err;
+    std::unique_ptr<tsr_solver> s(new tsr_solver());
+    s->opt = *opt;
+    if (opt->MaxKrylov < 1 || opt->MaxKrylov > kMaxKrylovCap) { set_error("MaxKrylov must be in [1, 120]"); return nullptr; }
+    s->mcap = kMaxKrylovCap;
+    s->P = upload_problem(*view, err);
+    if (!s->P) { set_error(err); return nullptr; }
+    if (hipStreamCreate(&s->st) != hipSuccess) { set_error("hipStreamCreate failed"); return nullptr; }
+    s->N = view->N;
+    s->S = view->n_scen;
+    const size_t N = (size_t)s->N;
+    const int S = s->S;
+    int rc = 0;
+    rc |= dalloc(&s->u, N); rc |= dalloc(&s->g0, N); rc |= dalloc(&s->u2, N); rc |= dalloc(&s->g2, N);
+    rc |= dalloc(&s->D, N); rc |= dalloc(&s->unew, N); rc |= dalloc(&s->scratch[0], N); rc |= dalloc(&s->scratch[1], N);
+    if (rc) { set_error("hipMalloc failed for the state vectors"); return nullptr; }
+    // Krylov bases: column 0 aliases the start vector.  Columns are allocated for the
+    // options' MaxKrylov; the learned controller may raise a scenario's cap up to 60,
+    // so allocate lazily up to mcap.
+    const int ncol = std::max(opt->MaxKrylov, kLearnedCap) + 1;
+    s->V.assign(s->mcap + 1, nullptr);
+    s->W.assign(s->mcap + 1, nullptr);
+    s->V[0] = s->g0;
+    s->W[0] = s->D;
+    for (int k = 1; k < ncol; ++k) {
+        if (dalloc(&s->V[k], N) || dalloc(&s->W[k], N)) { set_error("hipMalloc failed for the Krylov basis"); return nullptr; }
+    }
+    s->mcap = ncol - 1;
+    if (dalloc(&s->Vd, s->mcap + 1) || dalloc(&s->Wd, s->mcap + 1)) return nullptr;
+    hipMemcpy(s->Vd, s->V.data(), sizeof(double*) * (s->mcap + 1), hipMemcpyHostToDevice);
+    hipMemcpy(s->Wd, s->W.data(), sizeof(double*) * (s->mcap + 1), hipMemcpyHostToDevice);
+    // row blocks per scenario
+    const long long rpb = (long long)kVecBS * kVecR;
+    std::vector<int> bs, bf(S + 1, 0);
+    std::vector<long long> b0, b1;
+    s->rows_h.assign(S, 0);
+    for (int q = 0; q < S; ++q) {
+        const long long off = view->scen[q].u_offset, n = view->scen[q].ud.N;
+        s->rows_h[q] = n;
+        bf[q] = (int)bs.size();
+        for (long long r = 0; r < n; r += rpb) { bs.push_back(q); b0.push_back(off + r); b1.push_back(off + std::min(n, r + rpb)); }
+    }
+    bf[S] = (int)bs.size();
+    s->nblk = (int)bs.size();
+    int *d_bs = nullptr; long long *d_b0 = nullptr, *d_b1 = nullptr;
+    rc = dalloc(&d_bs, bs.size()) | dalloc(&d_b0, b0.size()) | dalloc(&d_b1, b1.size()) | dalloc(&s->blk_first, S + 1) |
+         dalloc(&s->scen_rows, S);
+    if (rc) return nullptr;
+    hipMemcpy(d_bs, bs.data(), sizeof(int) * bs.size(), hipMemcpyHostToDevice);
+    hipMemcpy(d_b0, b0.data(), sizeof(long long) * b0.size(), hipMemcpyHostToDevice);
+    hipMemcpy(d_b1, b1.data(), sizeof(long long) * b1.size(), hipMemcpyHostToDevice);
+    hipMemcpy(s->blk_first, bf.data(), sizeof(int) * (S + 1), hipMemcpyHostToDevice);
+    hipMemcpy(s->scen_rows, s->rows_h.data(), sizeof(long long) * S, hipMemcpyHostToDevice);
+    s->M.scen = d_bs; s->M.r0 = d_b0; s->M.r1 = d_b1;
+    s->ldp = s->mcap + 2;
+    if (dalloc(&s->part, (size_t)s->nblk * s->ldp)) return nullptr;
+    auto kstate = [&](KrylovState& K, double* shared_eps, double* shared_ieps) {
+        K.ld = s->mcap + 2;
+        K.ldh = s->mcap + 1;
+        K.hstride = (s->mcap + 1) * s->mcap;
+        int r = dalloc(&K.cs, (size_t)S * K.ld) | dalloc(&K.coef, (size_t)S * K.ld) | dalloc(&K.H, (size_t)S * K.hstride) |
+                dalloc(&K.veps, S) | dalloc(&K.res, (size_t)S * 8);
+        if (shared_eps) { K.eps = shared_eps; K.ieps = shared_ieps; }
+        else r |= dalloc(&K.eps, S) | dalloc(&K.ieps, S);
+        return r;
+    };
+    if (kstate(s->KV, nullptr, nullptr) || kstate(s->KW, s->KV.eps, s->KV.ieps)) { set_error("hipMalloc failed"); return nullptr; }
+    hipMemset(s->KV.cs, 0, sizeof(double) * S * s->KV.ld);
+    hipMemset(s->KW.cs, 0, sizeof(double) * S * s->KW.ld);
+    if (dalloc(&s->mask, S) || dalloc(&s->nk, S)) return nullptr;
+    s->bad = batch_of(s->P).bad;
+    s->mask_h.assign(S, 1); s->nk_h.assign(S, 0); s->bad_h.assign(S, 0);
+    s->coef_h.assign((size_t)S * s->KV.ld, 0.0);
+    s->res_h.assign((size_t)S * 8, 0.0);
+    s->sc.resize(S);
+    const int ldh = s->mcap + 1;
+    for (auto& c : s->sc) {
+        c.H.assign((size_t)ldh * s->mcap, 0.0);
+        c.HD.assign((size_t)ldh * s->mcap, 0.0);
+        c.phi.assign((size_t)s->mcap * s->mcap, 0.0);
+        c.y.assign(s->mcap + 1, 0.0);
+        c.z.assign(s->mcap + 1, 0.0);
+        c.cs.assign(s->mcap + 2, 0.0);
+        c.csW.assign(s->mcap + 2, 0.0);
+    }
+    hipEventCreate(&s->ev0);
+    hipEventCreate(&s->ev1);
+    tsr_solver_reset(s.get());
+    if (hipDeviceSynchronize() != hipSuccess) { set_error("device error during solver set-up"); return nullptr; }
+    return s.release();
+}
+
+}  // namespace
+
+extern "C" {
+
+tsr_solver* tsr_solver_create(const tsr_batch_view* view, const exprem_options_struct* opt)
+{
+    return create(view, opt);
+}
+
+void tsr_solver_free(tsr_solver* s)
+{
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        for (auto it = g_solver_cache.begin(); it != g_solver_cache.end();)
+            it = (it->second == s) ? g_solver_cache.erase(it) : std::next(it);
+    }
+    hipDeviceSynchronize();
+    for (double* p : {s->u, s->g0, s->u2, s->g2, s->D, s->unew, s->scratch[0], s->scratch[1]}) if (p) hipFree(p);
+    for (size_t k = 1; k < s->V.size(); ++k) { if (s->V[k]) hipFree(s->V[k]); if (s->W[k]) hipFree(s->W[k]); }
+    hipFree(s->Vd); hipFree(s->Wd);
+    hipFree((void*)s->M.scen); hipFree((void*)s->M.r0); hipFree((void*)s->M.r1);
+    hipFree(s->blk_first); hipFree(s->scen_rows); hipFree(s->part);
+    for (KrylovState* K : {&s->KV, &s->KW}) { hipFree(K->cs); hipFree(K->coef); hipFree(K->H); hipFree(K->veps); hipFree(K->res); }
+    hipFree(s->KV.eps); hipFree(s->KV.ieps);
+    hipFree(s->mask); hipFree(s->nk);
+    if (s->ev0) hipEventDestroy(s->ev0);
+    if (s->ev1) hipEventDestroy(s->ev1);
+    if (s->mlp) mlp_free(s->mlp);
+    free_problem(s->P);
+    if (s->st) hipStreamDestroy(s->st);
+    delete s;
+}
+
+void tsr_solver_reset(tsr_solver* s)
+{
+    for (auto& c : s->sc) {
+        c.t = 0.0;
+        c.proposal = s->opt.InitStep;
+        exprem_stats_init(&c.stats, &s->opt, nullptr);
+        for (int k = 0; k < 4; ++k) { c.hist_tau[k] = 0.0; c.hist_m[k] = 0.0; }
+    }
+}
+
+int tsr_solver_set_state(tsr_solver* s, const double* u_host, double t)
+{
+    const int fail_code = 2;
+    HIPCHK(hipMemcpy(s->u, u_host, sizeof(double) * s->N, hipMemcpyHostToDevice));
+    for (auto& c : s->sc) c.t = t;
+    return 0;
+}
+
+int tsr_solver_get_state(tsr_solver* s, double* u_host, double* t)
+{
+    const int fail_code = 2;
+    HIPCHK(hipStreamSynchronize(s->st));
+    HIPCHK(hipMemcpy(u_host, s->u, sizeof(double) * s->N, hipMemcpyDeviceToHost));
+    if (t) *t = s->sc[0].t;
+    return 0;
+}
+
+void tsr_solver_stats(const tsr_solver* s, exprem_stats_struct* st) { *st = s->sc[0].stats; }
+
+int tsr_solver_gfunc_host(tsr_solver* s, double* g, const double* u)
+{
+    const int fail_code = 2;
+    HIPCHK(hipMemcpyAsync(s->scratch[0], u, sizeof(double) * s->N, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMemsetAsync(s->bad, 0, sizeof(int) * s->S, s->st));
+    if (G_at(s, s->scratch[0], s->scratch[1], nullptr)) return 2;
+    HIPCHK(hipMemcpyAsync(g, s->scratch[1], sizeof(double) * s->N, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipMemcpyAsync(s->bad_h.data(), s->bad, sizeof(int) * s->S, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    for (int q = 0; q < s->S; ++q) if (s->bad_h[q]) return 1;
+    return 0;
+}
+
+int tsr_solver_gfunc_dev(tsr_solver* s, double* g_dev, const double* u_dev, int reps)
+{
+    const int fail_code = 2;
+    if (reps < 1) reps = 1;
+    HIPCHK(hipEventRecord(s->ev0, s->st));
+    for (int r = 0; r < reps; ++r)
+        if (G_at(s, u_dev, g_dev, nullptr)) return 2;
+    HIPCHK(hipEventRecord(s->ev1, s->st));
+    HIPCHK(hipEventSynchronize(s->ev1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    s->last_g_ms = ms / reps;
+    return 0;
+}
+
+double tsr_solver_last_gfunc_ms(const tsr_solver* s) { return s->last_g_ms; }
+
+int tsr_solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
+{
+    return solver_step(s, tend, rec);
+}
+
+int tsr_solver_run(tsr_solver* s, double tend, int max_steps, tsr_run_stats* out, tsr_step_record* records,
+                   int max_records)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    int steps = 0, rej = 0, fe = 0, rc = 0;
+    std::vector<tsr_step_record> rec(s->S);
+    for (; steps < max_steps; ++steps) {
+        bool done = true;
+        for (auto& c : s->sc) done = done && !(c.t < tend);
+        if (done) break;
+        rc = solver_step(s, tend, rec.data());
+        if (rc) break;
+        for (int q = 0; q < s->S; ++q) { rej += rec[q].rejects; fe += rec[q].fevals; }
+        if (records && steps * s->S < max_records)
+            for (int q = 0; q < s->S && steps * s->S + q < max_records; ++q) records[steps * s->S + q] = rec[q];
+    }
+    hipStreamSynchronize(s->st);
+    if (out) {
+        out->t = s->sc[0].t;
+        for (auto& c : s->sc) out->t = std::min(out->t, c.t);
+        out->steps = steps; out->rejects = rej; out->fevals = fe; out->failed = rc != 0;
+        out->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        out->gfunc_ms = 0.0; out->arnoldi_ms = 0.0;
+    }
+    return rc;
+}
+
+int tsr_solver_set_controller(tsr_solver* s, int mode)
+{
+    if (mode != 0 && mode != 1) return 1;
+    s->ctrl_mode = mode;
+    return 0;
+}
+
+int tsr_solver_set_mlp(tsr_solver* s, const float* params, const float* mean, const float* stdv)
+{
+    if (!s->mlp) s->mlp = mlp_create();
+    if (!s->mlp) return 2;
+    return mlp_set(s->mlp, params, mean, stdv);
+}
+
+double* tsr_solver_dev_u(tsr_solver* s) { return s->u; }
+double* tsr_solver_dev_scratch(tsr_solver* s, int which) { return s->scratch[which & 1]; }
+long long tsr_solver_N(const tsr_solver* s) { return s->N; }
+int tsr_device_sync(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : 2; }
+int tsr_h2d(double* dst, const double* src, long long n)
+{
+    return hipMemcpy(dst, src, sizeof(double) * n, hipMemcpyHostToDevice) == hipSuccess ? 0 : 2;
+}
+int tsr_d2h(double* dst, const double* src, long long n)
+{
+    return hipMemcpy(dst, src, sizeof(double) * n, hipMemcpyDeviceToHost) == hipSuccess ? 0 : 2;
+}
+
+// ---------------------------------------------------------------- reference signatures
+static tsr_solver* solver_for(const user_data_struct& ud, const exprem_options_struct* opt)
+{
+    const tsr_batch_view* v = registry_get(ud);
+    if (!v) { set_error("no problem registered for this user_data (use tsr_problem_*)"); return nullptr; }
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto it = g_solver_cache.find(v);
+    if (it != g_solver_cache.end() && (!opt || std::memcmp(&it->second->opt, opt, sizeof(*opt)) == 0)) return it->second;
+    exprem_options_struct o;
+    if (opt) o = *opt;
+    else if (it != g_solver_cache.end()) o = it->second->opt;
+    else { set_exprem_options(o); o.MaxKrylov = 30; }
+    tsr_solver* s = create(v, &o);
+    if (!s) return nullptr;
+    if (it != g_solver_cache.end()) {
+        // a caller changed options: keep the old solver alive (its device state may
+        // still be referenced by that caller's loop) and cache the new one
+        g_retired.push_back(it->second);
+        g_solver_cache.erase(it);
+    }
+    g_solver_cache[v] = s;
+    return s;
+}
+
+int Gfunc_hip(double* g, double* u, user_data_struct user_data)
+{
+    tsr_solver* s = solver_for(user_data, nullptr);
+    if (!s) return 2;
+    return tsr_solver_gfunc_host(s, g, u);
+}
+
+int exprem_c(tsr_gfunc_t Gfunc, int N, double* t, double tend, double* tspan, double* tspan_solns, double* u0,
+             user_data_struct ud, exprem_options_struct options, exprem_stats_struct* stats)
+{
+    tsr_solver* s = solver_for(ud, &options);
+    if (!s || s->N != N || s->S != 1) return -1;
+    if (*t == 0.0) exprem_stats_init(stats, &options, u0);
+    ScState& c = s->sc[0];
+    c.stats = *stats;
+    c.proposal = stats->CurrentStep;
+    c.t = *t;
+    s->host_g = (Gfunc == Gfunc_hip) ? nullptr : Gfunc;
+    if (s->host_g) {
+        s->host_ud = ud;
+        s->hx.resize(N); s->hg.resize(N); s->hg0.resize(N);
+    }
+    if (hipMemcpy(s->u, u0, sizeof(double) * N, hipMemcpyHostToDevice) != hipSuccess) return -1;
+    // land exactly on the next output time
+    double tstop = tend;
+    int out_k = -1;
+    const int nout = tspan ? (int)tspan[0] : 0;
+    for (int k = 1; k <= nout; ++k)
+        if (tspan[k] > *t * (1.0 + 1e-14) + 1e-300 && tspan[k] <= tstop) { tstop = tspan[k]; out_k = k; }
+    const int rc = solver_step(s, tstop, nullptr);
+    s->host_g = nullptr;
+    if (rc) return -1;
+    if (hipMemcpy(u0, s->u, sizeof(double) * N, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    *t = c.t;
+    if (out_k > 0 && tspan_solns && c.t == tstop) std::memcpy(tspan_solns + (size_t)out_k * N, u0, sizeof(double) * N);
+    *stats = c.stats;
+    stats->CurrentSoln = u0;
+    return 0;
+}
+
+double weighted_norm(double* e, double* u, double AbsTol, double RelTol, int N)
+{
+    double acc = 0.0;
+    for (int i = 0; i < N; ++i) {
+        const double r = e[i] / (AbsTol + RelTol * std::fabs(u[i]));
+        acc += r * r;
+    }
+    return std::sqrt(acc / (double)N);
+}
+
+}  // extern "C"
+
+int exprem(int (*Gfunc)(double* g, double* u, user_data_struct user_data), int N, double& t, double tend,
+           double* tspan, double* tspan_solns, double* u0, user_data_struct user_data, exprem_options_struct options,
+           exprem_stats_struct& stats)
+{
+    return exprem_c(Gfunc, N, &t, tend, tspan, tspan_solns, u0, user_data, options, &stats);
+}
