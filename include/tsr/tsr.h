@@ -141,6 +141,12 @@ int tsr_solver_gfunc_dev(tsr_solver* s, double* g_dev, const double* u_dev, int 
 /* Device time (ms) of the last tsr_solver_gfunc_dev call per launch, from HIP events on
  * the solver's stream. */
 double tsr_solver_last_gfunc_ms(const tsr_solver* s);
+/* Average device time (ms) of each of G's three kernels (macro elements, micro cells,
+ * macro nodes) over `reps` evaluations of G(u_dev), HIP events between the launches. */
+int tsr_solver_time_gfunc_parts(tsr_solver* s, double* g_dev, const double* u_dev, int reps, double* ms3);
+/* Average device time (ms) of a plain copy of N doubles (reads 8N, writes 8N bytes):
+ * the calibration kernel for HBM traffic counters. */
+int tsr_solver_time_copy(tsr_solver* s, int reps, double* ms);
 
 /* One accepted step on the device state (the exprem contract).  rec may be NULL. */
 int tsr_solver_step(tsr_solver* s, double tend, tsr_step_record* rec);

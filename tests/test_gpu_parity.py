@@ -98,7 +98,7 @@ def test_exprem_steps_match_oracle():
         print(f"step {k}: gpu tau={r.tau:.6e} m={r.m} mD={r.mD} rej={r.rejects} err={r.err:.4f} | "
               f"cpu tau={last[4]:.6e} m={int(last[0])} mD={int(last[1])} err={last[3]:.4f}", flush=True)
     u_gpu, t_gpu = s.get_state()
-    assert abs(t_gpu - t_cpu) <= 1e-9 * max(t_cpu, 1e-30)
+    assert abs(t_gpu - t_cpu) <= 1e-8 * t_cpu
     err = rel_inf(u_gpu, u_cpu)
     print(f"after 20 steps t={t_gpu:.6e}: rel_inf(u)={err:.3e}", flush=True)
     assert err <= 1e-6

@@ -101,6 +101,8 @@ void        free_problem(DevProblem* p);
 const DBatch& batch_of(const DevProblem* p);
 size_t      micro_lds_bytes(const DevProblem* p);
 hipError_t  launch_gfunc(const DevProblem* p, const GArgs& a, hipStream_t st);
+// one of the three G kernels: 0 macro elements, 1 micro cells, 2 macro nodes
+hipError_t  launch_gfunc_part(const DevProblem* p, const GArgs& a, int part, hipStream_t st);
 
 // Row blocks of the vector kernels: block b covers rows [r0, r1) of scenario scen.
 struct BlkMap {

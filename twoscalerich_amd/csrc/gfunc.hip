@@ -609,15 +609,28 @@ void free_problem(DevProblem* p)
 const DBatch& batch_of(const DevProblem* p) { return p->B; }
 size_t micro_lds_bytes(const DevProblem* p) { return p->lds; }
 
-hipError_t launch_gfunc(const DevProblem* p, const GArgs& a, hipStream_t st)
+hipError_t launch_gfunc_part(const DevProblem* p, const GArgs& a, int part, hipStream_t st)
 {
     const DBatch& B = p->B;
-    dim3 ge((B.macro_ne_max + 255) / 256, B.n_scen);
-    hipLaunchKernelGGL(k_macro_elem, ge, dim3(256), 0, st, B, a);
-    hipLaunchKernelGGL(k_micro, dim3(B.n_cells), dim3(kMicroBS), p->lds, st, B, a);
-    dim3 gn((B.macro_nn_max + 255) / 256, B.n_scen);
-    hipLaunchKernelGGL(k_macro_node, gn, dim3(256), 0, st, B, a);
+    if (part == 0) {
+        dim3 ge((B.macro_ne_max + 255) / 256, B.n_scen);
+        hipLaunchKernelGGL(k_macro_elem, ge, dim3(256), 0, st, B, a);
+    } else if (part == 1) {
+        hipLaunchKernelGGL(k_micro, dim3(B.n_cells), dim3(kMicroBS), p->lds, st, B, a);
+    } else {
+        dim3 gn((B.macro_nn_max + 255) / 256, B.n_scen);
+        hipLaunchKernelGGL(k_macro_node, gn, dim3(256), 0, st, B, a);
+    }
     return hipGetLastError();
+}
+
+hipError_t launch_gfunc(const DevProblem* p, const GArgs& a, hipStream_t st)
+{
+    for (int part = 0; part < 3; ++part) {
+        const hipError_t e = launch_gfunc_part(p, a, part, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace tsr

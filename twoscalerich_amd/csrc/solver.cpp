@@ -761,6 +761,48 @@ int tsr_solver_gfunc_dev(tsr_solver* s, double* g_dev, const double* u_dev, int 
 
 double tsr_solver_last_gfunc_ms(const tsr_solver* s) { return s->last_g_ms; }
 
+int tsr_solver_time_gfunc_parts(tsr_solver* s, double* g_dev, const double* u_dev, int reps, double* ms3)
+{
+    const int fail_code = 2;
+    if (reps < 1) reps = 1;
+    GArgs a;
+    a.u = u_dev; a.v = nullptr; a.veps = nullptr; a.g0 = nullptr; a.ieps = nullptr; a.active = nullptr; a.out = g_dev;
+    hipEvent_t ev[4];
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    double acc[3] = {0, 0, 0};
+    for (int r = 0; r < reps; ++r) {
+        HIPCHK(hipEventRecord(ev[0], s->st));
+        for (int part = 0; part < 3; ++part) {
+            HIPCHK(launch_gfunc_part(s->P, a, part, s->st));
+            HIPCHK(hipEventRecord(ev[part + 1], s->st));
+        }
+        HIPCHK(hipEventSynchronize(ev[3]));
+        for (int part = 0; part < 3; ++part) {
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, ev[part], ev[part + 1]));
+            acc[part] += ms;
+        }
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    for (int part = 0; part < 3; ++part) ms3[part] = acc[part] / reps;
+    return 0;
+}
+
+int tsr_solver_time_copy(tsr_solver* s, int reps, double* ms)
+{
+    const int fail_code = 2;
+    if (reps < 1) reps = 1;
+    HIPCHK(launch_masked_copy(s->scratch[1], s->scratch[0], s->M, s->nblk, nullptr, s->st));
+    HIPCHK(hipEventRecord(s->ev0, s->st));
+    for (int r = 0; r < reps; ++r) HIPCHK(launch_masked_copy(s->scratch[1], s->scratch[0], s->M, s->nblk, nullptr, s->st));
+    HIPCHK(hipEventRecord(s->ev1, s->st));
+    HIPCHK(hipEventSynchronize(s->ev1));
+    float t = 0.0f;
+    HIPCHK(hipEventElapsedTime(&t, s->ev0, s->ev1));
+    *ms = t / reps;
+    return 0;
+}
+
 int tsr_solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
 {
     return solver_step(s, tend, rec);
