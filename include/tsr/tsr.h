@@ -91,6 +91,10 @@ void                    tsr_problem_options(const tsr_problem* p, exprem_options
  * call is 16*N plus this. */
 long long               tsr_problem_distinct_bytes(const tsr_problem* p);
 
+/* 1 when a micro template is the uniform r x r right-triangle grid the analytic
+ * micro-cell kernel handles (connectivity, Dirichlet perimeter and geometry checked). */
+int tsr_template_structured(const micro_mesh_struct* m);
+
 const char* tsr_last_error(void);
 
 /* ============================== 3. device-resident solver ============================== */
@@ -121,6 +125,8 @@ typedef struct tsr_run_stats {
     int    steps, rejects, fevals, failed;
     double wall_s;
     double gfunc_ms, arnoldi_ms;   /* device time in the RHS and in the Krylov kernels */
+    double host_dense_s;           /* host time in the small dense phi routines */
+    double sync_wait_s;            /* host time blocked on the device stream */
 } tsr_run_stats;
 
 /* Creates device copies of the problem (mesh templates, soils, Keff tables) and the

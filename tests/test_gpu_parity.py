@@ -26,8 +26,14 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("generic", [False, True], ids=["analytic_grid", "generic_arrays"])
 @pytest.mark.parametrize("name,cfg,mn,mr,ns", CASES, ids=[c[0] for c in CASES])
-def test_gfunc_matches_oracle(name, cfg, mn, mr, ns):
+def test_gfunc_matches_oracle(name, cfg, mn, mr, ns, generic, monkeypatch):
+    """Both micro-cell kernels (analytic uniform grid, generic template arrays)."""
+    if generic:
+        monkeypatch.setenv("TSR_GENERIC_MICRO", "1")
+    else:
+        monkeypatch.delenv("TSR_GENERIC_MICRO", raising=False)
     p = T.Problem.config(cfg, mn, mr, ns)
     s = T.Solver(p)
     o = Oracle(p.view)

@@ -78,7 +78,8 @@ class tsr_step_record(C.Structure):
 
 class tsr_run_stats(C.Structure):
     _fields_ = [("t", C.c_double), ("steps", C.c_int), ("rejects", C.c_int), ("fevals", C.c_int),
-                ("failed", C.c_int), ("wall_s", C.c_double), ("gfunc_ms", C.c_double), ("arnoldi_ms", C.c_double)]
+                ("failed", C.c_int), ("wall_s", C.c_double), ("gfunc_ms", C.c_double), ("arnoldi_ms", C.c_double),
+                ("host_dense_s", C.c_double), ("sync_wait_s", C.c_double)]
 
 
 GFUNC_T = C.CFUNCTYPE(C.c_int, _dp, _dp, user_data_struct)
@@ -115,6 +116,7 @@ def declare(lib):
         "tsr_problem_options": (None, [P, C.POINTER(exprem_options_struct)]),
         "tsr_problem_distinct_bytes": (C.c_longlong, [P]),
         "tsr_last_error": (C.c_char_p, []),
+        "tsr_template_structured": (C.c_int, [C.POINTER(micro_mesh_struct)]),
         "tsr_solver_create": (P, [C.POINTER(tsr_batch_view), C.POINTER(exprem_options_struct)]),
         "tsr_solver_free": (None, [P]),
         "tsr_solver_set_state": (C.c_int, [P, _dp, C.c_double]),

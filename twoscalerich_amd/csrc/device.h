@@ -13,8 +13,7 @@
 
 namespace tsr {
 
-constexpr int kMicroBS = 256;         // threads per micro-cell workgroup
-constexpr int kMicroNPT = 4;          // nodes per thread: templates up to 1024 nodes
+constexpr int kMicroMaxBS = 1024;     // micro-cell workgroup: one thread per node, up to 1024 nodes
 constexpr int kMaxNodeInc = 12;       // macro elements around a node hosting a cell
 constexpr int kVecBS = 256;           // threads per workgroup of the vector kernels
 constexpr int kVecR = 4;              // rows per thread of the vector kernels
@@ -36,6 +35,14 @@ struct DTmpl {
     const int* bnd;                   // [nn]
     const unsigned char* soil;        // [ne] or nullptr
     double cell_area;
+    // Uniform r x r right-triangle grid (the generator's layout: node j*r+i, square
+    // (i,j) split into A = (a, a+1, a+r+1) and B = (a, a+r+1, a+r), element 2*(j*(r-1)+i)
+    // + {0,1}, Dirichlet nodes = perimeter): geometry of one A and one B element serves
+    // every element.  sg[t] = sf x-coeffs (3), y-coeffs (3), normals (6), sub-face
+    // K weights (9, j*3+v), SCV area (1).
+    int structured;
+    int r;
+    double sg[2][22];
 };
 
 struct DScen {

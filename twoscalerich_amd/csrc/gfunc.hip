@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -158,21 +159,59 @@ __device__ __forceinline__ int elem_soil(const DBatch& B, const DTmpl& T, int ce
     return T.soil ? (int)T.soil[e] : TSR_SOIL_B;
 }
 
-__global__ __launch_bounds__(kMicroBS) void k_micro(DBatch B, GArgs a)
+// Visit the incident (element, local vertex) pairs of node i in ascending element
+// order (the order the oracle's CSR gives).  Generic templates read the CSR; the
+// structured grid calls f on the (at most six) pairs of node (ix, iy) in straight-line
+// code, so nothing is indexed dynamically and everything stays in registers.
+template <bool STRUCT, class F>
+__device__ __forceinline__ void for_incident(const DTmpl& T, int i, F&& f)
+{
+    if constexpr (STRUCT) {
+        const int r = T.r, rm = r - 1, ix = i % r, iy = i / r;
+        const bool L = ix >= 1, R = ix < rm, D = iy >= 1, U = iy < rm;
+        const int sLD = 2 * ((iy - 1) * rm + ix - 1), sRD = 2 * ((iy - 1) * rm + ix);
+        const int sLU = 2 * (iy * rm + ix - 1), sRU = 2 * (iy * rm + ix);
+        if (L && D) { f(sLD, 2); f(sLD + 1, 1); }
+        if (R && D) f(sRD + 1, 2);
+        if (L && U) f(sLU, 1);
+        if (R && U) { f(sRU, 0); f(sRU + 1, 0); }
+    } else {
+        const int q1 = T.ptr[i + 1];
+        for (int q = T.ptr[i]; q < q1; ++q) {
+            const int v = T.inc[q];
+            f(v >> 2, v & 3);
+        }
+    }
+}
+
+template <bool STRUCT>
+__device__ __forceinline__ bool is_dirichlet(const DTmpl& T, int i)
+{
+    if constexpr (STRUCT) {
+        const int ix = i % T.r, iy = i / T.r;
+        return ix == 0 || iy == 0 || ix == T.r - 1 || iy == T.r - 1;
+    } else {
+        return T.bnd[i] == 1;
+    }
+}
+
+template <bool STRUCT>
+__global__ __launch_bounds__(kMicroMaxBS) void k_micro(DBatch B, GArgs a)
 {
     extern __shared__ __align__(16) double smem[];
     __shared__ double s_bc[6];
     __shared__ double s_mel[kMaxNodeInc][12];
     __shared__ int s_ninc;
-    __shared__ double s_red[kMicroBS / 64];
-
+    __shared__ double s_red[kMicroMaxBS / 64];
+    __shared__ double s_geo[2][22];
     const int cell = blockIdx.x;
     const int s = B.cell_scen[cell];
     if (a.active && !a.active[s]) return;
     const int t = B.cell_tmpl[cell];
     const DTmpl& T = B.tmpl[t];
     const DScen& S = B.scen[s];
-    const int nn = T.nn, ne = T.ne, tid = threadIdx.x;
+    const int nn = T.nn, ne = T.ne, tid = threadIdx.x, bs = blockDim.x;
+    const DSoil soilA = S.A, soilB = S.B;   // uniform: scalar registers
     double* h = smem;
     double* KA = h + B.nn_max;
     double* KB = KA + B.nn_max;
@@ -209,12 +248,15 @@ __global__ __launch_bounds__(kMicroBS) void k_micro(DBatch B, GArgs a)
             for (int v = 0; v < 3; ++v) s_mel[tid][9 + v] = macro_head(S, a, ve, S.el[v * S.ne + e]);
         }
     }
+    if constexpr (STRUCT) {
+        if (tid < 44) s_geo[tid / 22][tid % 22] = T.sg[tid / 22][tid % 22];
+    }
     __syncthreads();
 
     // heads (Dirichlet nodes from the macro field) and element soils
-    for (int i = tid; i < nn; i += kMicroBS) {
+    for (int i = tid; i < nn; i += bs) {
         double hi;
-        if (T.bnd[i] == 1) {
+        if (is_dirichlet<STRUCT>(T, i)) {
             if (node_host) {
                 if (model == 1) {
                     hi = s_bc[3];
@@ -251,53 +293,82 @@ __global__ __launch_bounds__(kMicroBS) void k_micro(DBatch B, GArgs a)
         }
         h[i] = hi;
     }
-    for (int e = tid; e < ne; e += kMicroBS) soil[e] = (unsigned char)elem_soil(B, T, cell, e);
+    for (int e = tid; e < ne; e += bs) soil[e] = (unsigned char)elem_soil(B, T, cell, e);
     __syncthreads();
 
-    // node-centric constitutive evaluation: storage kept in registers for the gather
-    double stor[kMicroNPT];
-#pragma unroll
-    for (int k = 0; k < kMicroNPT; ++k) {
-        stor[k] = 1.0;
-        const int i = tid + k * kMicroBS;
+    // node-centric constitutive evaluation (one node per thread: the workgroup has at
+    // least nn threads); storage kept in a register for the gather
+    double stor = 1.0;
+    {
+        const int i = tid;
         if (i < nn) {
             double cva = 0.0, cvb = 0.0;
-            for (int qq = T.ptr[i]; qq < T.ptr[i + 1]; ++qq) {
-                const int e = T.inc[qq] >> 2;
-                if (soil[e] == TSR_SOIL_A) cva += T.scv[e]; else cvb += T.scv[e];
-            }
+            for_incident<STRUCT>(T, i, [&](int e, int) {
+                const double sc = STRUCT ? s_geo[e & 1][21] : T.scv[e];
+                if (soil[e] == TSR_SOIL_A) cva += sc; else cvb += sc;
+            });
             double cA = 0.0, cB = 0.0, kA = 0.0, kB = 0.0;
             // first evaluation with the lane's own soil (no divergence), second only at interfaces
             const bool hasA = cva > 0.0, hasB = cvb > 0.0;
             {
+                DSoil sl;
+                sl.Ks = hasA ? soilA.Ks : soilB.Ks; sl.tr = hasA ? soilA.tr : soilB.tr;
+                sl.ts = hasA ? soilA.ts : soilB.ts; sl.al = hasA ? soilA.al : soilB.al;
+                sl.n = hasA ? soilA.n : soilB.n; sl.m = hasA ? soilA.m : soilB.m;
                 double C, K;
-                vg_ck(hasA ? S.A : S.B, h[i], B.Ss, C, K);
+                vg_ck(sl, h[i], B.Ss, C, K);
                 if (hasA) { cA = C; kA = K; } else { cB = C; kB = K; }
             }
             if (hasA && hasB) {
                 double C, K;
-                vg_ck(S.B, h[i], B.Ss, C, K);
+                vg_ck(soilB, h[i], B.Ss, C, K);
                 cB = C; kB = K;
             }
             KA[i] = kA;
             KB[i] = kB;
-            stor[k] = cA * cva + cB * cvb;
+            stor = cA * cva + cB * cvb;
         }
     }
     __syncthreads();
 
     // sub-face fluxes: q[e*3+j] leaves v_j, enters v_{j+1}
-    for (int e = tid; e < ne; e += kMicroBS) {
-        const int v0 = T.el[e], v1 = T.el[ne + e], v2 = T.el[2 * ne + e];
+    for (int e = tid; e < ne; e += bs) {
+        int v0, v1, v2;
+        const double* g;   // structured: per-type geometry
+        if constexpr (STRUCT) {
+            const int sq = e >> 1, r = T.r;
+            const int a0 = (sq / (r - 1)) * r + sq % (r - 1);
+            v0 = a0;
+            v1 = (e & 1) ? a0 + r + 1 : a0 + 1;
+            v2 = (e & 1) ? a0 + r : a0 + r + 1;
+            g = s_geo[e & 1];
+        } else {
+            v0 = T.el[e]; v1 = T.el[ne + e]; v2 = T.el[2 * ne + e];
+            g = nullptr;
+        }
         const double h0 = h[v0], h1 = h[v1], h2 = h[v2];
-        const double gx = T.sf[e] * h0 + T.sf[ne + e] * h1 + T.sf[2 * ne + e] * h2;
-        const double gy = T.sf[3 * ne + e] * h0 + T.sf[4 * ne + e] * h1 + T.sf[5 * ne + e] * h2;
+        double gx, gy;
+        if constexpr (STRUCT) {
+            gx = g[0] * h0 + g[1] * h1 + g[2] * h2;
+            gy = g[3] * h0 + g[4] * h1 + g[5] * h2;
+        } else {
+            gx = T.sf[e] * h0 + T.sf[ne + e] * h1 + T.sf[2 * ne + e] * h2;
+            gy = T.sf[3 * ne + e] * h0 + T.sf[4 * ne + e] * h1 + T.sf[5 * ne + e] * h2;
+        }
         const double* Ks = (soil[e] == TSR_SOIL_A) ? KA : KB;
         const double k0 = Ks[v0], k1 = Ks[v1], k2 = Ks[v2];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            const double Kf = T.w[(j * 3) * ne + e] * k0 + T.w[(j * 3 + 1) * ne + e] * k1 + T.w[(j * 3 + 2) * ne + e] * k2;
-            const double nx = T.nrm[(2 * j) * ne + e], ny = T.nrm[(2 * j + 1) * ne + e];
+            double Kf, nx, ny;
+            if constexpr (STRUCT) {
+                Kf = g[12 + j * 3] * k0 + g[12 + j * 3 + 1] * k1 + g[12 + j * 3 + 2] * k2;
+                nx = g[6 + 2 * j];
+                ny = g[6 + 2 * j + 1];
+            } else {
+                Kf = T.w[(j * 3) * ne + e] * k0 + T.w[(j * 3 + 1) * ne + e] * k1 + T.w[(j * 3 + 2) * ne + e] * k2;
+                nx = T.nrm[(2 * j) * ne + e];
+                ny = T.nrm[(2 * j + 1) * ne + e];
+            }
             q[e * 3 + j] = -Kf * (gx * nx + (gy + 1.0) * ny);
         }
     }
@@ -305,19 +376,17 @@ __global__ __launch_bounds__(kMicroBS) void k_micro(DBatch B, GArgs a)
 
     // node-centric gather (incident order = element order, identical on every run)
     double Qacc = 0.0;
-#pragma unroll
-    for (int k = 0; k < kMicroNPT; ++k) {
-        const int i = tid + k * kMicroBS;
+    {
+        const int i = tid;
         if (i < nn) {
             double net = 0.0;
-            for (int qq = T.ptr[i]; qq < T.ptr[i + 1]; ++qq) {
-                const int e = T.inc[qq] >> 2, lv = T.inc[qq] & 3;
+            for_incident<STRUCT>(T, i, [&](int e, int lv) {
                 net += q[e * 3 + (lv + 2) % 3];
                 net -= q[e * 3 + lv];
-            }
+            });
             double gi;
-            if (T.bnd[i] == 1) gi = 0.0;
-            else { gi = net / stor[k]; Qacc += net; }
+            if (is_dirichlet<STRUCT>(T, i)) gi = 0.0;
+            else { gi = net / stor; Qacc += net; }
             store_out(B, a, s, off + i, gi);
         }
     }
@@ -326,7 +395,7 @@ __global__ __launch_bounds__(kMicroBS) void k_micro(DBatch B, GArgs a)
     __syncthreads();
     if (tid == 0) {
         double Q = 0.0;
-        for (int w = 0; w < kMicroBS / 64; ++w) Q += s_red[w];
+        for (int w = 0; w < bs / 64; ++w) Q += s_red[w];
         B.Q[cell] = Q;
     }
 }
@@ -377,6 +446,7 @@ struct DevProblem {
     size_t bytes = 0;
     DBatch B{};
     size_t lds = 0;
+    bool all_structured = false;      // every template is a uniform grid: analytic k_micro
 };
 
 namespace {
@@ -415,10 +485,72 @@ void csr(int nn, int ne, int nv, const int* el, std::vector<int>& ptr, std::vect
 
 }  // namespace
 
+// A template is the generator's uniform r x r right-triangle grid when its
+// connectivity, Dirichlet set and every element's geometry (to 1e-12) match the
+// pattern; then sg[0], sg[1] hold the A and B element geometry for the analytic kernel.
+static bool detect_structured(const micro_mesh_struct& m, const double* w, const double* scv, double sg[2][22])
+{
+    const int nn = m.no_nodes, ne = m.no_elements;
+    const int r = (int)std::lround(std::sqrt((double)nn));
+    bool st = r >= 2 && r * r == nn && ne == 2 * (r - 1) * (r - 1);
+    for (int sq = 0; st && sq < (r - 1) * (r - 1); ++sq) {
+        const int i = sq % (r - 1), jj = sq / (r - 1), a0 = jj * r + i;
+        const int eA = 2 * sq, eB = eA + 1;
+        st = m.elements[eA] == a0 && m.elements[ne + eA] == a0 + 1 && m.elements[2 * ne + eA] == a0 + r + 1 &&
+             m.elements[eB] == a0 && m.elements[ne + eB] == a0 + r + 1 && m.elements[2 * ne + eB] == a0 + r;
+    }
+    for (int i = 0; st && i < nn; ++i) {
+        const int ix = i % r, iy = i / r;
+        const bool per = ix == 0 || ix == r - 1 || iy == 0 || iy == r - 1;
+        st = (m.boundary_nodes[i] == 1) == per;
+    }
+    auto geo = [&](int e, double* g) {
+        for (int k = 0; k < 6; ++k) g[k] = m.shape_funcs[k * ne + e];
+        for (int k = 0; k < 6; ++k) g[6 + k] = m.normals[k * ne + e];
+        for (int k = 0; k < 9; ++k) g[12 + k] = w[(size_t)k * ne + e];
+        g[21] = scv[e];
+    };
+    if (st) {
+        geo(0, sg[0]);
+        geo(1, sg[1]);
+        double g[22];
+        for (int e = 0; st && e < ne; ++e) {
+            geo(e, g);
+            const double* ref = sg[e & 1];
+            for (int k = 0; k < 22 && st; ++k) st = std::fabs(g[k] - ref[k]) <= 1e-12 * std::max(1.0, std::fabs(ref[k]));
+        }
+    }
+    return st;
+}
+
+// Sub-face K weights and SCV areas of a template (the arrays k_micro<false> reads).
+static void template_weights(const micro_mesh_struct& m, std::vector<double>& w, std::vector<double>& scv)
+{
+    const int nn = m.no_nodes, ne = m.no_elements;
+    w.assign((size_t)9 * ne, 0.0);
+    scv.assign(ne, 0.0);
+    for (int e = 0; e < ne; ++e) {
+        const int a0 = m.elements[e], b0 = m.elements[ne + e], c0 = m.elements[2 * ne + e];
+        const double xa = m.nodes[a0], ya = m.nodes[nn + a0];
+        const double xb = m.nodes[b0], yb = m.nodes[nn + b0];
+        const double xc = m.nodes[c0], yc = m.nodes[nn + c0];
+        const double det = (xb - xa) * (yc - ya) - (xc - xa) * (yb - ya);
+        scv[e] = std::fabs(det) / 6.0;
+        for (int j = 0; j < 3; ++j) {
+            const double mx = m.midx[j * ne + e], my = m.midy[j * ne + e];
+            for (int vv = 0; vv < 3; ++vv)
+                w[(size_t)(j * 3 + vv) * ne + e] = m.shape_funcs[(0 * 3 + vv) * ne + e] * mx +
+                                                   m.shape_funcs[(1 * 3 + vv) * ne + e] * my +
+                                                   m.shape_funcs[(2 * 3 + vv) * ne + e];
+        }
+    }
+}
+
 DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
 {
     Arena A;
-    struct TOff { size_t ptr, inc, el, sf, nrm, w, scv, vec, cen, bnd, soil; int nn, ne; double ca; bool has_soil; };
+    struct TOff { size_t ptr, inc, el, sf, nrm, w, scv, vec, cen, bnd, soil; int nn, ne; double ca; bool has_soil;
+                  int structured, r; double sg[2][22]; };
     struct SOff { size_t keff, el, nodes, sf, nrm, midx, midy, cv, scv, bce, blen, bmid, bnrm, cen, ptr, inc, dir, alpha, qe, bq; };
     std::vector<TOff> to(v.n_tmpl);
     std::vector<SOff> so(v.n_scen);
@@ -427,29 +559,20 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
         const micro_mesh_struct& m = v.tmpl[t];
         if (m.no_verts != 3) { err = "micro templates must be triangles"; return nullptr; }
         const int nn = m.no_nodes, ne = m.no_elements;
-        if (nn > kMicroBS * kMicroNPT) { err = "micro template has more than 1024 nodes"; return nullptr; }
+        if (nn > kMicroMaxBS) { err = "micro template has more than 1024 nodes"; return nullptr; }
         nn_max = std::max(nn_max, nn);
         ne_max = std::max(ne_max, ne);
         std::vector<int> ptr, inc;
         csr(nn, ne, 3, m.elements, ptr, inc);
-        std::vector<double> w((size_t)9 * ne), scv(ne);
-        for (int e = 0; e < ne; ++e) {
-            const int a0 = m.elements[e], b0 = m.elements[ne + e], c0 = m.elements[2 * ne + e];
-            const double xa = m.nodes[a0], ya = m.nodes[nn + a0];
-            const double xb = m.nodes[b0], yb = m.nodes[nn + b0];
-            const double xc = m.nodes[c0], yc = m.nodes[nn + c0];
-            const double det = (xb - xa) * (yc - ya) - (xc - xa) * (yb - ya);
-            scv[e] = std::fabs(det) / 6.0;
-            for (int j = 0; j < 3; ++j) {
-                const double mx = m.midx[j * ne + e], my = m.midy[j * ne + e];
-                for (int vv = 0; vv < 3; ++vv)
-                    w[(size_t)(j * 3 + vv) * ne + e] = m.shape_funcs[(0 * 3 + vv) * ne + e] * mx +
-                                                       m.shape_funcs[(1 * 3 + vv) * ne + e] * my +
-                                                       m.shape_funcs[(2 * 3 + vv) * ne + e];
-            }
-        }
+        std::vector<double> w, scv;
+        template_weights(m, w, scv);
         TOff& o = to[t];
         o.nn = nn; o.ne = ne; o.ca = m.cell_area;
+        o.structured = 0; o.r = 0;
+        if (detect_structured(m, w.data(), scv.data(), o.sg)) {
+            o.structured = 1;
+            o.r = (int)std::lround(std::sqrt((double)nn));
+        }
         o.ptr = A.put(ptr.data(), ptr.size());
         o.inc = A.put(inc.data(), inc.size());
         o.el = A.put(m.elements, (size_t)3 * ne);
@@ -542,6 +665,9 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
         T.scv = at<double>(base, o.scv); T.vec = at<double>(base, o.vec); T.cen = at<double>(base, o.cen);
         T.bnd = at<int>(base, o.bnd);
         T.soil = o.has_soil ? at<unsigned char>(base, o.soil) : nullptr;
+        T.structured = o.structured;
+        T.r = o.r;
+        std::memcpy(T.sg, o.sg, sizeof(T.sg));
         std::memcpy(A.host.data() + o_T + t * sizeof(DTmpl), &T, sizeof(DTmpl));
     }
     for (int s = 0; s < v.n_scen; ++s) {
@@ -586,10 +712,13 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
     B.discs = v.max_discs > 0 ? at<double>(base, o_disc) : nullptr;
     B.Q = at<double>(base, o_Q); B.bad = at<int>(base, o_bad);
     B.Ss = v.Ss; B.N = v.N;
+    P->all_structured = !std::getenv("TSR_GENERIC_MICRO");
+    for (int t = 0; t < v.n_tmpl; ++t) P->all_structured = P->all_structured && to[t].structured;
     P->lds = ((size_t)3 * nn_max + (size_t)3 * ne_max) * sizeof(double) + (size_t)ne_max;
     P->lds = (P->lds + 15) & ~size_t(15);
     if (P->lds > 64 * 1024) {
-        if (hipFuncSetAttribute((const void*)k_micro, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds) != hipSuccess) {
+        if (hipFuncSetAttribute((const void*)k_micro<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds) != hipSuccess ||
+            hipFuncSetAttribute((const void*)k_micro<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds) != hipSuccess) {
             err = "micro template needs more LDS than a workgroup may allocate";
             hipFree(P->buf);
             delete P;
@@ -616,7 +745,11 @@ hipError_t launch_gfunc_part(const DevProblem* p, const GArgs& a, int part, hipS
         dim3 ge((B.macro_ne_max + 255) / 256, B.n_scen);
         hipLaunchKernelGGL(k_macro_elem, ge, dim3(256), 0, st, B, a);
     } else if (part == 1) {
-        hipLaunchKernelGGL(k_micro, dim3(B.n_cells), dim3(kMicroBS), p->lds, st, B, a);
+        const int bs = std::min(kMicroMaxBS, std::max(64, (B.nn_max + 63) / 64 * 64));
+        if (p->all_structured)
+            hipLaunchKernelGGL(k_micro<true>, dim3(B.n_cells), dim3(bs), p->lds, st, B, a);
+        else
+            hipLaunchKernelGGL(k_micro<false>, dim3(B.n_cells), dim3(bs), p->lds, st, B, a);
     } else {
         dim3 gn((B.macro_nn_max + 255) / 256, B.n_scen);
         hipLaunchKernelGGL(k_macro_node, gn, dim3(256), 0, st, B, a);
@@ -634,3 +767,13 @@ hipError_t launch_gfunc(const DevProblem* p, const GArgs& a, hipStream_t st)
 }
 
 }  // namespace tsr
+
+// 1 when the micro-cell kernel can use the analytic uniform-grid path for this template.
+extern "C" int tsr_template_structured(const micro_mesh_struct* m)
+{
+    std::vector<double> w, scv;
+    if (m->no_verts != 3) return 0;
+    tsr::template_weights(*m, w, scv);
+    double sg[2][22];
+    return tsr::detect_structured(*m, w.data(), scv.data(), sg) ? 1 : 0;
+}

@@ -5,8 +5,10 @@
 // (kVecR rows per thread, strided by the workgroup size so each load instruction is
 // one contiguous 2 KiB segment).  Reductions are two-stage and deterministic: per
 // workgroup a wavefront shuffle tree and a fixed-order sum over its 4 wavefronts into
-// part[b], then k_reduce sums a scenario's workgroups in ascending order.  No FP64
-// atomics anywhere, so every rerun is bitwise identical.
+// part[k * nblk + b] (column-major, so the second stage reads contiguously), then
+// k_reduce gives each (scenario, column) its own workgroup, which sums that column's
+// partials with a fixed strided assignment and a fixed tree.  No FP64 atomics
+// anywhere, so every rerun is bitwise identical.
 //
 // Basis columns are stored unnormalised; v_k = cs[k] * raw_k.  The scale is folded
 // into the coefficients by k_reduce, so no pass ever rescales a column.
@@ -38,9 +40,10 @@ __device__ __forceinline__ double block_sum(double x, double* red)
     return s;
 }
 
-// mode PASS_DOTS:       part[b][k] = sum_rows col_k * w                     (k < nk), part[b][nk] = sum w^2
+// part[k][b] below is part[k * ldp + b] (ldp = number of workgroups).
+// mode PASS_DOTS:       part[k][b] = sum_rows col_k * w                     (k < nk), part[nk][b] = sum w^2
 // mode PASS_AXPY_DOTS:  w -= sum_k col_k * coef[s][k]; then as PASS_DOTS (w written back)
-// mode PASS_AXPY_NORM:  w -= sum_k col_k * coef[s][k]; part[b][0] = sum w^2, part[b][1] = sum (w/wt)^2,
+// mode PASS_AXPY_NORM:  w -= sum_k col_k * coef[s][k]; part[0][b] = sum w^2, part[1][b] = sum (w/wt)^2,
 //                       wt = at + rt |u|
 __global__ __launch_bounds__(kVecBS) void k_pass(int mode, const double* const* cols, int nk, double* w,
                                                 const double* u, const double* coef, int ldc, BlkMap M,
@@ -86,10 +89,34 @@ __global__ __launch_bounds__(kVecBS) void k_pass(int mode, const double* const* 
             }
         s0 = block_sum(s0, red);
         s1 = block_sum(s1, red);
-        if (tid == 0) { part[(size_t)b * ldp] = s0; part[(size_t)b * ldp + 1] = s1; }
+        if (tid == 0) { part[b] = s0; part[(size_t)ldp + b] = s1; }
         return;
     }
-    for (int k = 0; k < nk; ++k) {
+    // four columns at a time: 16 independent loads in flight per thread, four
+    // independent shuffle trees
+    int k = 0;
+    for (; k + 4 <= nk; k += 4) {
+        const double* c0 = cols[k];
+        const double* c1 = cols[k + 1];
+        const double* c2 = cols[k + 2];
+        const double* c3 = cols[k + 3];
+        double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+#pragma unroll
+        for (int r = 0; r < kVecR; ++r)
+            if (ok[r]) {
+                const long long i = idx[r];
+                p0 += c0[i] * wr[r];
+                p1 += c1[i] * wr[r];
+                p2 += c2[i] * wr[r];
+                p3 += c3[i] * wr[r];
+            }
+        p0 = wave_sum_v(p0);
+        p1 = wave_sum_v(p1);
+        p2 = wave_sum_v(p2);
+        p3 = wave_sum_v(p3);
+        if (lane == 0) { lds[wv][k] = p0; lds[wv][k + 1] = p1; lds[wv][k + 2] = p2; lds[wv][k + 3] = p3; }
+    }
+    for (; k < nk; ++k) {
         const double* col = cols[k];
         double p = 0.0;
 #pragma unroll
@@ -106,14 +133,14 @@ __global__ __launch_bounds__(kVecBS) void k_pass(int mode, const double* const* 
         if (lane == 0) lds[wv][nk] = p;
     }
     __syncthreads();
-    for (int k = tid; k <= nk; k += kVecBS) {
+    for (int kk = tid; kk <= nk; kk += kVecBS) {
         double acc = 0.0;
-        for (int x = 0; x < kVecBS / 64; ++x) acc += lds[x][k];
-        part[(size_t)b * ldp + k] = acc;
+        for (int x = 0; x < kVecBS / 64; ++x) acc += lds[x][kk];
+        part[(size_t)kk * ldp + b] = acc;
     }
 }
 
-// part[b] = [sum a^2, sum (a/wt(u))^2, sum u^2]
+// part[0..2][b] = sum a^2, sum (a/wt(u))^2, sum u^2
 __global__ __launch_bounds__(kVecBS) void k_norms(const double* a, const double* u, BlkMap M, double* part, int ldp,
                                                  const int* active, double at, double rt)
 {
@@ -137,11 +164,11 @@ __global__ __launch_bounds__(kVecBS) void k_norms(const double* a, const double*
     s0 = block_sum(s0, red);
     s1 = block_sum(s1, red);
     s2 = block_sum(s2, red);
-    if (threadIdx.x == 0) { part[(size_t)b * ldp] = s0; part[(size_t)b * ldp + 1] = s1; part[(size_t)b * ldp + 2] = s2; }
+    if (threadIdx.x == 0) { part[b] = s0; part[(size_t)ldp + b] = s1; part[(size_t)2 * ldp + b] = s2; }
 }
 
 // out = base + sgn * sum_{k < nk_s[s]} col_k * coef[s][k] - extra_sub
-// partmode 0: nothing; 1: part[b][0] = sum out^2; 2: part[b][0] = sum (c/wt(u))^2 of the
+// partmode 0: nothing; 1: part[0][b] = sum out^2; 2: part[0][b] = sum (c/wt(u))^2 of the
 // combination c = sum col_k coef_k alone (the error vector) with out = base + c.
 __global__ __launch_bounds__(kVecBS) void k_lincomb(double* out, const double* base, const double* const* cols,
                                                    const int* nk_s, const double* coef, int ldc, double sgn, BlkMap M,
@@ -184,7 +211,7 @@ __global__ __launch_bounds__(kVecBS) void k_lincomb(double* out, const double* b
     }
     if (partmode) {
         s0 = block_sum(s0, red);
-        if (threadIdx.x == 0) part[(size_t)b * ldp] = s0;
+        if (threadIdx.x == 0) part[b] = s0;
     }
 }
 
@@ -201,67 +228,83 @@ __global__ __launch_bounds__(kVecBS) void k_masked_copy(double* dst, const doubl
     }
 }
 
-// One workgroup per scenario: sum the scenario's workgroup partials in ascending
-// order, then finish the Gram-Schmidt bookkeeping:
-//   RED_H1:   h1_k = cs_k sum_k; H[k][j] = h1_k; coef_k = cs_k h1_k; res[1] = sum_nk (||w||^2 before CGS)
+// Second stage: workgroup (s, k) sums column k of scenario s's workgroup partials
+// (thread t takes partials t, t+256, ... in order, then a fixed tree), then finishes
+// the Gram-Schmidt bookkeeping for that column:
+//   RED_H1:   h1_k = cs_k sum_k; H[k][j] = h1_k; coef_k = cs_k h1_k; (k = nk) res[1] = ||w||^2 before CGS
 //   RED_H2:   h2_k = cs_k sum_k; H[k][j] += h2_k; coef_k = cs_k h2_k
-//   RED_NORM: n = sqrt(sum_0); H[j+1][j] = n; cs[j+1] = 1/n; veps = eps cs[j+1];
-//             res[0] = n; res[2] = sqrt(sum_1/N_s)/n  (WRMS of v_{j+1})
-//   RED_START (column 0 = start vector): beta = sqrt(sum_0); cs[0] = 1/beta; res[0] = beta;
-//             res[3] = ||u||_2; eps = sqrt(DBL_EPSILON)(1 + ||u||_2) unless j < 0 (keep eps);
-//             ieps = 1/eps; veps = eps cs[0]
-//   RED_SUM:  res[k] = sum_k (k < nk)
+//   RED_NORM: (k = 0) n = sqrt(sum_0); H[j+1][j] = n; cs[j+1] = 1/n; veps = eps cs[j+1]; res[0] = n;
+//             (k = 1) res[2] = sqrt(sum_1 / N_s) (WRMS of w; divided by n on the host)
+//   RED_START (column 0 = start vector): (k = 0) beta = sqrt(sum_0); cs[0] = 1/beta; res[0] = beta;
+//             (k = 2) res[3] = ||u||_2; eps = sqrt(DBL_EPSILON)(1 + ||u||_2) unless j < 0; ieps = 1/eps;
+//             then k_start_finish sets veps = eps cs[0]
+//   RED_SUM:  res[k] = sum_k
 __global__ __launch_bounds__(256) void k_reduce(int mode, const double* part, int ldp, int nk, const int* blk_first,
                                                 const long long* scen_rows, KrylovState K, int j, const int* active)
 {
-    __shared__ double acc[kMaxCols + 2];
+    __shared__ double red[256 / 64];
     const int s = blockIdx.x;
+    const int k = blockIdx.y;
     if (active && !active[s]) return;
-    const int b0 = blk_first[s], b1 = blk_first[s + 1];
     const int nsum = (mode == RED_H1 || mode == RED_H2) ? nk + 1 : nk;
-    for (int k = threadIdx.x; k < nsum; k += blockDim.x) {
-        double x = 0.0;
-        for (int b = b0; b < b1; ++b) x += part[(size_t)b * ldp + k];
-        acc[k] = x;
-    }
+    if (k >= nsum) return;
+    const int b0 = blk_first[s], b1 = blk_first[s + 1];
+    const double* col = part + (size_t)k * ldp;
+    double x = 0.0;
+    for (int b = b0 + threadIdx.x; b < b1; b += 256) x += col[b];
+    x = wave_sum_v(x);
+    const int tid = threadIdx.x;
+    if ((tid & 63) == 0) red[tid >> 6] = x;
     __syncthreads();
+    if (tid != 0) return;
+    double acc = 0.0;
+    for (int w = 0; w < 4; ++w) acc += red[w];
     double* cs = K.cs + (size_t)s * K.ld;
     double* coef = K.coef + (size_t)s * K.ld;
     double* H = K.H + (size_t)s * K.hstride;
     double* res = K.res + (size_t)s * 8;
-    const int tid = threadIdx.x;
     if (mode == RED_H1 || mode == RED_H2) {
-        for (int k = tid; k < nk; k += blockDim.x) {
-            const double hk = cs[k] * acc[k];
-            if (mode == RED_H1) H[(size_t)j * K.ldh + k] = hk;
-            else H[(size_t)j * K.ldh + k] += hk;
-            coef[k] = cs[k] * hk;
+        if (k == nk) {
+            if (mode == RED_H1) res[1] = acc;
+            return;
         }
-        if (tid == 0 && mode == RED_H1) res[1] = acc[nk];
+        const double hk = cs[k] * acc;
+        if (mode == RED_H1) H[(size_t)j * K.ldh + k] = hk;
+        else H[(size_t)j * K.ldh + k] += hk;
+        coef[k] = cs[k] * hk;
     } else if (mode == RED_NORM) {
-        if (tid == 0) {
-            const double n = sqrt(acc[0]);
+        if (k == 0) {
+            const double n = sqrt(acc);
             H[(size_t)j * K.ldh + j + 1] = n;
             cs[j + 1] = 1.0 / n;
             K.veps[s] = K.eps[s] * cs[j + 1];
             res[0] = n;
-            res[2] = sqrt(acc[1] / (double)scen_rows[s]) / n;
+        } else if (k == 1) {
+            res[2] = sqrt(acc / (double)scen_rows[s]);
         }
     } else if (mode == RED_START) {
-        if (tid == 0) {
-            const double beta = sqrt(acc[0]);
+        if (k == 0) {
+            const double beta = sqrt(acc);
             cs[0] = 1.0 / beta;
             res[0] = beta;
-            res[3] = sqrt(acc[2]);
+        } else if (k == 2) {
+            res[3] = sqrt(acc);
             if (j >= 0) {
                 K.eps[s] = sqrt(DBL_EPSILON) * (1.0 + res[3]);
                 K.ieps[s] = 1.0 / K.eps[s];
             }
-            K.veps[s] = K.eps[s] * cs[0];
         }
     } else {
-        for (int k = tid; k < nk; k += blockDim.x) res[k] = acc[k];
+        res[k] = acc;
     }
+}
+
+// veps = eps cs[0] once both RED_START columns are done (one thread per scenario)
+__global__ void k_start_finish(KrylovState K, int n_scen, const int* active)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_scen || (active && !active[s])) return;
+    K.veps[s] = K.eps[s] * K.cs[(size_t)s * K.ld];
 }
 
 // ---------------------------------------------------------------- launchers
@@ -294,7 +337,11 @@ hipError_t launch_lincomb(double* out, const double* base, const double* const* 
 hipError_t launch_reduce(int mode, const double* part, int ldp, int nk, const int* blk_first, int n_scen,
                          const long long* scen_rows, KrylovState K, int j, const int* active, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_reduce, dim3(n_scen), dim3(256), 0, st, mode, part, ldp, nk, blk_first, scen_rows, K, j, active);
+    const int ncol = (mode == RED_H1 || mode == RED_H2) ? nk + 1 : nk;
+    hipLaunchKernelGGL(k_reduce, dim3(n_scen, ncol), dim3(256), 0, st, mode, part, ldp, nk, blk_first, scen_rows, K, j,
+                       active);
+    if (mode == RED_START)
+        hipLaunchKernelGGL(k_start_finish, dim3((n_scen + 255) / 256), dim3(256), 0, st, K, n_scen, active);
     return hipGetLastError();
 }
 

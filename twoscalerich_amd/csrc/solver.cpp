@@ -60,6 +60,8 @@ struct ScState {
     double beta = 0.0, hn = 0.0, hnD = 0.0, betaD = 0.0, err = 0.0, eps = 0.0, wnext = 0.0, wnextD = 0.0;
     bool bd = false, bdD = false, conv = false, failed = false, accepted = false;
     std::vector<double> H, HD, phi, y, z, cs, csW;
+    double phi_tau = -1.0;    // (tau, m) of the phi_1 matrix currently in phi
+    int phi_m = -1;
     exprem_stats_struct stats{};
     // recent history for the learned controller
     double hist_tau[4] = {0, 0, 0, 0};
@@ -106,7 +108,29 @@ struct tsr_solver {
     // timing of the last run
     double g_ms_acc = 0.0;
     int    g_calls = 0;
+    double* pin_h = nullptr;  // pinned host buffers for per-iteration readbacks
+    double* pin_res = nullptr;
+    int* pin_bad = nullptr;
+    double t_dense = 0.0;     // host seconds in the small dense phi routines
+    double t_sync = 0.0;      // host seconds waiting for the stream
 };
+
+namespace {
+struct Tic {
+    double& acc;
+    std::chrono::steady_clock::time_point t0;
+    explicit Tic(double& a) : acc(a), t0(std::chrono::steady_clock::now()) {}
+    ~Tic() { acc += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+};
+tsr_solver* g_timed = nullptr;   // solver whose dense/sync time is being accumulated
+hipError_t timed_sync(tsr_solver* s)
+{
+    Tic tic(s->t_sync);
+    return hipStreamSynchronize(s->st);
+}
+double dense_dummy = 0.0;
+double& dense_acc() { return g_timed ? g_timed->t_dense : dense_dummy; }
+}
 
 namespace {
 
@@ -136,7 +160,7 @@ int G_call(tsr_solver* s, const double* v, const double* veps, const double* g0,
             HIPCHK(hipMemcpyAsync(&ve, veps, sizeof(double), hipMemcpyDeviceToHost, s->st));
         }
         if (g0) HIPCHK(hipMemcpyAsync(&ie, ieps, sizeof(double), hipMemcpyDeviceToHost, s->st));
-        HIPCHK(hipStreamSynchronize(s->st));
+        HIPCHK(timed_sync(s));
         if (v) for (size_t i = 0; i < N; ++i) s->hx[i] += ve * vh[i];
         const int flag = s->host_g(s->hg.data(), s->hx.data(), s->host_ud);
         if (g0) for (size_t i = 0; i < N; ++i) s->hg[i] = (s->hg[i] - s->hg0[i]) * ie;
@@ -191,19 +215,24 @@ int read_iteration(tsr_solver* s, KrylovState& K, int j, std::vector<double>* Hc
         HIPCHK(hipMemcpy2DAsync(tmp.data(), sizeof(double) * (j + 2), K.H + (size_t)j * K.ldh,
                                 sizeof(double) * K.hstride, sizeof(double) * (j + 2), s->S, hipMemcpyDeviceToHost,
                                 s->st));
-        HIPCHK(hipStreamSynchronize(s->st));
+        HIPCHK(timed_sync(s));
         for (int q = 0; q < s->S; ++q)
             if (s->mask_h[q])
                 for (int i = 0; i <= j + 1; ++i) Hcols[q][(size_t)j * K.ldh + i] = tmp[(size_t)q * (j + 2) + i];
     } else {
-        HIPCHK(hipStreamSynchronize(s->st));
+        HIPCHK(timed_sync(s));
     }
     return 0;
 }
 
 double krylov_est_main(ScState& c, int m, double tau, int ldh, int ldphi)
 {
+    double dummy = 0.0;
+    Tic tic(g_timed ? g_timed->t_dense : dummy);
+    c.phi_m = -1;
     if (phi1_pade(c.phi.data(), c.H.data(), tau, m, ldh, ldphi)) return INFINITY;
+    c.phi_tau = tau;
+    c.phi_m = m;
     return tau * (tau * c.beta * c.hn * std::fabs(c.phi[m - 1])) * c.wnext;
 }
 
@@ -221,7 +250,7 @@ int arnoldi(tsr_solver* s, int which)
     for (int q = 0; q < s->S; ++q) {
         ScState& c = s->sc[q];
         Hs[q] = which == 0 ? &c.H : &c.HD;
-        if (which == 0) { c.m = 0; c.bd = false; c.conv = false; } else { c.mD = 0; c.bdD = false; }
+        if (which == 0) { c.m = 0; c.bd = false; c.conv = false; c.phi_m = -1; } else { c.mD = 0; c.bdD = false; }
     }
     std::vector<std::vector<double>> Hv(s->S);
     for (int j = 0; j < s->mcap; ++j) {
@@ -242,23 +271,20 @@ int arnoldi(tsr_solver* s, int which)
         HIPCHK(launch_pass(PASS_AXPY_DOTS, Bd, j + 1, B[j + 1], s->u, K.coef, K.ld, s->M, s->nblk, s->part, s->ldp,
                            s->mask, s->opt.AbsTol, s->opt.RelTol, s->st));
         HIPCHK(launch_reduce(RED_H2, s->part, s->ldp, j + 1, s->blk_first, s->S, s->scen_rows, K, j, s->mask, s->st));
-        // the ||w||^2 slot of the first projection is needed for the breakdown test: keep it
-        std::vector<double> w0(s->S, 0.0);
-        HIPCHK(hipMemcpy2DAsync(w0.data(), sizeof(double), K.res + 1, sizeof(double) * 8, sizeof(double), s->S,
-                                hipMemcpyDeviceToHost, s->st));
         HIPCHK(launch_pass(PASS_AXPY_NORM, Bd, j + 1, B[j + 1], s->u, K.coef, K.ld, s->M, s->nblk, s->part, s->ldp,
                            s->mask, s->opt.AbsTol, s->opt.RelTol, s->st));
         HIPCHK(launch_reduce(RED_NORM, s->part, s->ldp, 2, s->blk_first, s->S, s->scen_rows, K, j, s->mask, s->st));
-        std::vector<double> hbuf;
-        // gather H column j for every active scenario
+        // gather H column j, the per-scenario results and the RHS flags of this iteration
         {
-            std::vector<double> tmp((size_t)s->S * (j + 2));
-            HIPCHK(hipMemcpy2DAsync(tmp.data(), sizeof(double) * (j + 2), K.H + (size_t)j * K.ldh,
+            double* tmp = s->pin_h;
+            HIPCHK(hipMemcpy2DAsync(tmp, sizeof(double) * (j + 2), K.H + (size_t)j * K.ldh,
                                     sizeof(double) * K.hstride, sizeof(double) * (j + 2), s->S,
                                     hipMemcpyDeviceToHost, s->st));
-            HIPCHK(hipMemcpyAsync(s->res_h.data(), K.res, sizeof(double) * 8 * s->S, hipMemcpyDeviceToHost, s->st));
-            HIPCHK(hipMemcpyAsync(s->bad_h.data(), s->bad, sizeof(int) * s->S, hipMemcpyDeviceToHost, s->st));
-            HIPCHK(hipStreamSynchronize(s->st));
+            HIPCHK(hipMemcpyAsync(s->pin_res, K.res, sizeof(double) * 8 * s->S, hipMemcpyDeviceToHost, s->st));
+            HIPCHK(hipMemcpyAsync(s->pin_bad, s->bad, sizeof(int) * s->S, hipMemcpyDeviceToHost, s->st));
+            HIPCHK(timed_sync(s));
+            std::memcpy(s->res_h.data(), s->pin_res, sizeof(double) * 8 * s->S);
+            std::memcpy(s->bad_h.data(), s->pin_bad, sizeof(int) * s->S);
             for (int q = 0; q < s->S; ++q)
                 if (s->mask_h[q])
                     for (int i = 0; i <= j + 1; ++i) (*Hs[q])[(size_t)j * ldh + i] = tmp[(size_t)q * (j + 2) + i];
@@ -269,8 +295,8 @@ int arnoldi(tsr_solver* s, int which)
             c.fe++;
             if (s->bad_h[q]) { c.failed = true; s->mask_h[q] = 0; continue; }
             const double hn = s->res_h[(size_t)q * 8 + 0];
-            const double wn = s->res_h[(size_t)q * 8 + 2];
-            const double wnorm0 = std::sqrt(w0[q]);
+            const double wn = s->res_h[(size_t)q * 8 + 2] / hn;
+            const double wnorm0 = std::sqrt(s->res_h[(size_t)q * 8 + 1]);
             const int m = j + 1;
             if (which == 0) {
                 c.m = m; c.hn = hn; c.wnext = wn;
@@ -286,7 +312,12 @@ int arnoldi(tsr_solver* s, int which)
                 if (!(hn > 1e-12 * wnorm0) || hn == 0.0) { c.bdD = true; s->mask_h[q] = 0; continue; }
                 if (m >= s->opt.MinKrylov) {
                     std::vector<double> ph(3 * (size_t)m);
-                    if (!phik_e1(ph.data(), c.HD.data(), ldh, m, c.tau, 3)) {
+                    int prc;
+                    {
+                        Tic tic(dense_acc());
+                        prc = phik_e1(ph.data(), c.HD.data(), ldh, m, c.tau, 3);
+                    }
+                    if (!prc) {
                         const double est = 2.0 * c.tau * c.betaD * (c.tau * hn * std::fabs(ph[2 * m + m - 1])) * wn;
                         if (est <= kKrylovTol) s->mask_h[q] = 0;
                     }
@@ -380,7 +411,13 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
             std::fill(cf, cf + s->KV.ld, 0.0);
             s->nk_h[q] = 0;
             if (!pend[q] || !(c.beta > 0.0)) continue;
-            if (phi1_pade(c.phi.data(), c.H.data(), c.tau, c.m, ldh, s->mcap)) return -1;
+            if (!(c.phi_m == c.m && c.phi_tau == c.tau)) {
+                Tic tic(dense_acc());
+                c.phi_m = -1;
+                if (phi1_pade(c.phi.data(), c.H.data(), c.tau, c.m, ldh, s->mcap)) return -1;
+                c.phi_tau = c.tau;
+                c.phi_m = c.m;
+            }
             for (int k = 0; k < c.m; ++k) { c.y[k] = c.tau * c.beta * c.phi[k]; cf[k] = c.cs[k] * c.y[k]; }
             s->nk_h[q] = c.m;
         }
@@ -438,7 +475,7 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
             // csW[0] and veps for the D basis
             std::vector<double> csrow((size_t)S * s->KW.ld, 0.0), vep(S, 0.0);
             HIPCHK(hipMemcpyAsync(csrow.data(), s->KW.cs, sizeof(double) * S * s->KW.ld, hipMemcpyDeviceToHost, s->st));
-            HIPCHK(hipStreamSynchronize(s->st));
+            HIPCHK(timed_sync(s));
             for (int q = 0; q < S; ++q) {
                 if (!dq[q]) continue;
                 csrow[(size_t)q * s->KW.ld] = s->sc[q].csW[0];
@@ -459,7 +496,10 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
             s->nk_h[q] = 0;
             if (!pend[q] || !dq[q]) continue;
             std::vector<double> ph(3 * (size_t)c.mD);
-            if (phik_e1(ph.data(), c.HD.data(), ldh, c.mD, c.tau, 3)) return -1;
+            {
+                Tic tic(dense_acc());
+                if (phik_e1(ph.data(), c.HD.data(), ldh, c.mD, c.tau, 3)) return -1;
+            }
             for (int k = 0; k < c.mD; ++k) { c.z[k] = 2.0 * c.tau * c.betaD * ph[2 * c.mD + k]; cf[k] = c.csW[k] * c.z[k]; }
             s->nk_h[q] = c.mD;
         }
@@ -630,8 +670,8 @@ tsr_solver* create(const tsr_batch_view* view, const exprem_options_struct* opt)
     hipMemcpy(s->blk_first, bf.data(), sizeof(int) * (S + 1), hipMemcpyHostToDevice);
     hipMemcpy(s->scen_rows, s->rows_h.data(), sizeof(long long) * S, hipMemcpyHostToDevice);
     s->M.scen = d_bs; s->M.r0 = d_b0; s->M.r1 = d_b1;
-    s->ldp = s->mcap + 2;
-    if (dalloc(&s->part, (size_t)s->nblk * s->ldp)) return nullptr;
+    s->ldp = s->nblk;   // partials are column-major: part[k * nblk + b], k < mcap + 2
+    if (dalloc(&s->part, (size_t)s->nblk * (s->mcap + 2))) return nullptr;
     auto kstate = [&](KrylovState& K, double* shared_eps, double* shared_ieps) {
         K.ld = s->mcap + 2;
         K.ldh = s->mcap + 1;
@@ -660,6 +700,12 @@ tsr_solver* create(const tsr_batch_view* view, const exprem_options_struct* opt)
         c.z.assign(s->mcap + 1, 0.0);
         c.cs.assign(s->mcap + 2, 0.0);
         c.csW.assign(s->mcap + 2, 0.0);
+    }
+    if (hipHostMalloc((void**)&s->pin_h, sizeof(double) * (size_t)S * (s->mcap + 2)) != hipSuccess ||
+        hipHostMalloc((void**)&s->pin_res, sizeof(double) * (size_t)S * 8) != hipSuccess ||
+        hipHostMalloc((void**)&s->pin_bad, sizeof(int) * (size_t)S) != hipSuccess) {
+        set_error("hipHostMalloc failed");
+        return nullptr;
     }
     hipEventCreate(&s->ev0);
     hipEventCreate(&s->ev1);
@@ -694,6 +740,9 @@ void tsr_solver_free(tsr_solver* s)
     for (KrylovState* K : {&s->KV, &s->KW}) { hipFree(K->cs); hipFree(K->coef); hipFree(K->H); hipFree(K->veps); hipFree(K->res); }
     hipFree(s->KV.eps); hipFree(s->KV.ieps);
     hipFree(s->mask); hipFree(s->nk);
+    if (s->pin_h) hipHostFree(s->pin_h);
+    if (s->pin_res) hipHostFree(s->pin_res);
+    if (s->pin_bad) hipHostFree(s->pin_bad);
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
     if (s->mlp) mlp_free(s->mlp);
@@ -813,6 +862,9 @@ int tsr_solver_run(tsr_solver* s, double tend, int max_steps, tsr_run_stats* out
 {
     const auto t0 = std::chrono::steady_clock::now();
     int steps = 0, rej = 0, fe = 0, rc = 0;
+    s->t_dense = 0.0;
+    s->t_sync = 0.0;
+    g_timed = s;
     std::vector<tsr_step_record> rec(s->S);
     for (; steps < max_steps; ++steps) {
         bool done = true;
@@ -831,7 +883,10 @@ int tsr_solver_run(tsr_solver* s, double tend, int max_steps, tsr_run_stats* out
         out->steps = steps; out->rejects = rej; out->fevals = fe; out->failed = rc != 0;
         out->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         out->gfunc_ms = 0.0; out->arnoldi_ms = 0.0;
+        out->host_dense_s = s->t_dense;
+        out->sync_wait_s = s->t_sync;
     }
+    g_timed = nullptr;
     return rc;
 }
 
