@@ -42,6 +42,9 @@ struct DTmpl {
     // K weights (9, j*3+v), SCV area (1).
     int structured;
     int r;
+    unsigned mul_r, mul_rm;           // n / r == __umulhi(n, mul_r), n / (r-1) == __umulhi(n, mul_rm), n < 2^16
+    int nbnd;                         // Dirichlet nodes (generic templates: listed in bnd_list)
+    const int* bnd_list;              // [nbnd] (generic templates)
     double sg[2][22];
 };
 

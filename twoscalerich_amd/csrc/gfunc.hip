@@ -163,11 +163,30 @@ __device__ __forceinline__ int elem_soil(const DBatch& B, const DTmpl& T, int ce
 // order (the order the oracle's CSR gives).  Generic templates read the CSR; the
 // structured grid calls f on the (at most six) pairs of node (ix, iy) in straight-line
 // code, so nothing is indexed dynamically and everything stays in registers.
+__device__ __forceinline__ int div_r(const DTmpl& T, int n) { return (int)__umulhi((unsigned)n, T.mul_r); }
+__device__ __forceinline__ int div_rm(const DTmpl& T, int n) { return (int)__umulhi((unsigned)n, T.mul_rm); }
+
+// Node of the b-th Dirichlet node: structured grids walk the perimeter (bottom row,
+// right column, top row, left column); generic templates use their list.
+template <bool STRUCT>
+__device__ __forceinline__ int dirichlet_node(const DTmpl& T, int b)
+{
+    if constexpr (STRUCT) {
+        const int rm = T.r - 1;
+        if (b < rm) return b;                                   // (b, 0)
+        if (b < 2 * rm) return (b - rm) * T.r + rm;             // (r-1, b-rm)
+        if (b < 3 * rm) return rm * T.r + (3 * rm - b);         // (3rm-b, r-1)
+        return (4 * rm - b) * T.r;                              // (0, 4rm-b)
+    } else {
+        return T.bnd_list[b];
+    }
+}
+
 template <bool STRUCT, class F>
 __device__ __forceinline__ void for_incident(const DTmpl& T, int i, F&& f)
 {
     if constexpr (STRUCT) {
-        const int r = T.r, rm = r - 1, ix = i % r, iy = i / r;
+        const int r = T.r, rm = r - 1, iy = div_r(T, i), ix = i - iy * r;
         const bool L = ix >= 1, R = ix < rm, D = iy >= 1, U = iy < rm;
         const int sLD = 2 * ((iy - 1) * rm + ix - 1), sRD = 2 * ((iy - 1) * rm + ix);
         const int sLU = 2 * (iy * rm + ix - 1), sRU = 2 * (iy * rm + ix);
@@ -188,7 +207,7 @@ template <bool STRUCT>
 __device__ __forceinline__ bool is_dirichlet(const DTmpl& T, int i)
 {
     if constexpr (STRUCT) {
-        const int ix = i % T.r, iy = i / T.r;
+        const int iy = div_r(T, i), ix = i - iy * T.r;
         return ix == 0 || iy == 0 || ix == T.r - 1 || iy == T.r - 1;
     } else {
         return T.bnd[i] == 1;
@@ -253,46 +272,46 @@ __global__ __launch_bounds__(kMicroMaxBS) void k_micro(DBatch B, GArgs a)
     }
     __syncthreads();
 
-    // heads (Dirichlet nodes from the macro field) and element soils
-    for (int i = tid; i < nn; i += bs) {
+    // heads: Dirichlet nodes from the macro field (first nbnd threads), the rest from u
+    const int nbnd = STRUCT ? 4 * (T.r - 1) : T.nbnd;
+    for (int b = tid; b < nbnd; b += bs) {
+        const int i = dirichlet_node<STRUCT>(T, b);
         double hi;
-        if (is_dirichlet<STRUCT>(T, i)) {
-            if (node_host) {
-                if (model == 1) {
-                    hi = s_bc[3];
-                } else {
-                    const double px = fmin(fmax(s_bc[4] + T.vec[i], S.box[0]), S.box[1]);
-                    const double py = fmin(fmax(s_bc[5] + T.vec[nn + i], S.box[2]), S.box[3]);
-                    int best = -1;
-                    double bmin = -INFINITY, lam[3] = {0, 0, 0};
-                    for (int k = 0; k < s_ninc; ++k) {
-                        double l[3], mn = INFINITY;
-#pragma unroll
-                        for (int v = 0; v < 3; ++v) {
-                            l[v] = s_mel[k][v] * px + s_mel[k][3 + v] * py + s_mel[k][6 + v];
-                            mn = fmin(mn, l[v]);
-                        }
-                        if (best < 0 || mn > bmin + 1e-12) { bmin = mn; best = k; lam[0] = l[0]; lam[1] = l[1]; lam[2] = l[2]; }
-                    }
-                    double ws = 0.0, hs = 0.0;
+        if (node_host) {
+            if (model == 1) {
+                hi = s_bc[3];
+            } else {
+                const double px = fmin(fmax(s_bc[4] + T.vec[i], S.box[0]), S.box[1]);
+                const double py = fmin(fmax(s_bc[5] + T.vec[nn + i], S.box[2]), S.box[3]);
+                int best = -1;
+                double bmin = -INFINITY, lam[3] = {0, 0, 0};
+                for (int k = 0; k < s_ninc; ++k) {
+                    double l[3], mn = INFINITY;
 #pragma unroll
                     for (int v = 0; v < 3; ++v) {
-                        const double w = fmax(lam[v], 0.0);
-                        ws += w;
-                        hs += w * s_mel[best][9 + v];
+                        l[v] = s_mel[k][v] * px + s_mel[k][3 + v] * py + s_mel[k][6 + v];
+                        mn = fmin(mn, l[v]);
                     }
-                    hi = hs / ws;
+                    if (best < 0 || mn > bmin + 1e-12) { bmin = mn; best = k; lam[0] = l[0]; lam[1] = l[1]; lam[2] = l[2]; }
                 }
-            } else {
-                const double X = (model == 1) ? s_bc[4] : s_bc[4] + T.vec[i];
-                const double Y = (model == 1) ? s_bc[5] : s_bc[5] + T.vec[nn + i];
-                hi = s_bc[0] * X + s_bc[1] * Y + s_bc[2] * X * Y + s_bc[3];
+                double ws = 0.0, hs = 0.0;
+#pragma unroll
+                for (int v = 0; v < 3; ++v) {
+                    const double w = fmax(lam[v], 0.0);
+                    ws += w;
+                    hs += w * s_mel[best][9 + v];
+                }
+                hi = hs / ws;
             }
         } else {
-            hi = ld_state(a, ve, off + i);
+            const double X = (model == 1) ? s_bc[4] : s_bc[4] + T.vec[i];
+            const double Y = (model == 1) ? s_bc[5] : s_bc[5] + T.vec[nn + i];
+            hi = s_bc[0] * X + s_bc[1] * Y + s_bc[2] * X * Y + s_bc[3];
         }
         h[i] = hi;
     }
+    for (int i = tid; i < nn; i += bs)
+        if (!is_dirichlet<STRUCT>(T, i)) h[i] = ld_state(a, ve, off + i);
     for (int e = tid; e < ne; e += bs) soil[e] = (unsigned char)elem_soil(B, T, cell, e);
     __syncthreads();
 
@@ -337,7 +356,8 @@ __global__ __launch_bounds__(kMicroMaxBS) void k_micro(DBatch B, GArgs a)
         const double* g;   // structured: per-type geometry
         if constexpr (STRUCT) {
             const int sq = e >> 1, r = T.r;
-            const int a0 = (sq / (r - 1)) * r + sq % (r - 1);
+            const int row = div_rm(T, sq);
+            const int a0 = row * r + (sq - row * (r - 1));
             v0 = a0;
             v1 = (e & 1) ? a0 + r + 1 : a0 + 1;
             v2 = (e & 1) ? a0 + r : a0 + r + 1;
@@ -549,8 +569,8 @@ static void template_weights(const micro_mesh_struct& m, std::vector<double>& w,
 DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
 {
     Arena A;
-    struct TOff { size_t ptr, inc, el, sf, nrm, w, scv, vec, cen, bnd, soil; int nn, ne; double ca; bool has_soil;
-                  int structured, r; double sg[2][22]; };
+    struct TOff { size_t ptr, inc, el, sf, nrm, w, scv, vec, cen, bnd, soil, blist; int nn, ne; double ca; bool has_soil;
+                  int structured, r, nbnd; unsigned mul_r, mul_rm; double sg[2][22]; };
     struct SOff { size_t keff, el, nodes, sf, nrm, midx, midy, cv, scv, bce, blen, bmid, bnrm, cen, ptr, inc, dir, alpha, qe, bq; };
     std::vector<TOff> to(v.n_tmpl);
     std::vector<SOff> so(v.n_scen);
@@ -568,11 +588,21 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
         template_weights(m, w, scv);
         TOff& o = to[t];
         o.nn = nn; o.ne = ne; o.ca = m.cell_area;
-        o.structured = 0; o.r = 0;
+        o.structured = 0; o.r = 0; o.mul_r = 0; o.mul_rm = 0;
         if (detect_structured(m, w.data(), scv.data(), o.sg)) {
-            o.structured = 1;
-            o.r = (int)std::lround(std::sqrt((double)nn));
+            const int r = (int)std::lround(std::sqrt((double)nn));
+            const unsigned mr = (unsigned)(0xFFFFFFFFull / (unsigned)r) + 1u;
+            const unsigned mrm = (unsigned)(0xFFFFFFFFull / (unsigned)(r - 1)) + 1u;
+            bool exact = r >= 3;
+            for (unsigned n = 0; exact && n < 4u * (unsigned)nn; ++n)
+                exact = (unsigned)(((unsigned long long)n * mr) >> 32) == n / (unsigned)r &&
+                        (unsigned)(((unsigned long long)n * mrm) >> 32) == n / (unsigned)(r - 1);
+            if (exact) { o.structured = 1; o.r = r; o.mul_r = mr; o.mul_rm = mrm; }
         }
+        std::vector<int> blist;
+        for (int i = 0; i < nn; ++i) if (m.boundary_nodes[i] == 1) blist.push_back(i);
+        o.nbnd = (int)blist.size();
+        o.blist = A.put(blist.data(), blist.size());
         o.ptr = A.put(ptr.data(), ptr.size());
         o.inc = A.put(inc.data(), inc.size());
         o.el = A.put(m.elements, (size_t)3 * ne);
@@ -667,6 +697,10 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
         T.soil = o.has_soil ? at<unsigned char>(base, o.soil) : nullptr;
         T.structured = o.structured;
         T.r = o.r;
+        T.mul_r = o.mul_r;
+        T.mul_rm = o.mul_rm;
+        T.nbnd = o.nbnd;
+        T.bnd_list = at<int>(base, o.blist);
         std::memcpy(T.sg, o.sg, sizeof(T.sg));
         std::memcpy(A.host.data() + o_T + t * sizeof(DTmpl), &T, sizeof(DTmpl));
     }
