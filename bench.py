@@ -123,6 +123,8 @@ def main():
     micro_bytes = 16.0 * n_micro + p.distinct_bytes()
     g_bytes = 16.0 * p.N + p.distinct_bytes()
     micro_gbs = micro_bytes / (ms3[1] * 1e-3) / 1e9
+    shape = (C.c_int(), C.c_int())
+    L.tsr_solver_micro_shape(s.h, C.byref(shape[0]), C.byref(shape[1]))
     copy_ms = C.c_double()
     L.tsr_solver_time_copy(s.h, args.kernel_reps, C.byref(copy_ms))
     copy_gbs = 16.0 * p.N / (copy_ms.value * 1e-3) / 1e9
@@ -174,7 +176,8 @@ def main():
         "roofline": {"kernel": "k_micro", "bound": "hbm", "achieved": round(micro_gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(micro_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes": micro_bytes, "avg_launch_ms": round(ms3[1], 4),
-                     "copy_kernel_GB_per_s": round(copy_gbs, 1)},
+                     "copy_kernel_GB_per_s": round(copy_gbs, 1),
+                     "launch_shape": {"workgroup": shape[0].value, "waves_per_eu": shape[1].value}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

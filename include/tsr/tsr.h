@@ -147,6 +147,9 @@ int tsr_solver_gfunc_dev(tsr_solver* s, double* g_dev, const double* u_dev, int 
 /* Device time (ms) of the last tsr_solver_gfunc_dev call per launch, from HIP events on
  * the solver's stream. */
 double tsr_solver_last_gfunc_ms(const tsr_solver* s);
+/* Launch shape the autotuner chose for the micro-cell kernel on this problem
+ * (workgroup size, waves-per-EU register budget; 0 before the first G). */
+void tsr_solver_micro_shape(const tsr_solver* s, int* bs, int* wpe);
 /* Average device time (ms) of each of G's three kernels (macro elements, micro cells,
  * macro nodes) over `reps` evaluations of G(u_dev), HIP events between the launches. */
 int tsr_solver_time_gfunc_parts(tsr_solver* s, double* g_dev, const double* u_dev, int reps, double* ms3);

@@ -128,6 +128,7 @@ def declare(lib):
         "tsr_solver_last_gfunc_ms": (C.c_double, [P]),
         "tsr_solver_time_gfunc_parts": (C.c_int, [P, C.c_void_p, C.c_void_p, C.c_int, _dp]),
         "tsr_solver_time_copy": (C.c_int, [P, C.c_int, _dp]),
+        "tsr_solver_micro_shape": (None, [P, _ip, _ip]),
         "tsr_solver_step": (C.c_int, [P, C.c_double, C.POINTER(tsr_step_record)]),
         "tsr_solver_run": (C.c_int, [P, C.c_double, C.c_int, C.POINTER(tsr_run_stats),
                                      C.POINTER(tsr_step_record), C.c_int]),

@@ -13,7 +13,8 @@
 
 namespace tsr {
 
-constexpr int kMicroMaxBS = 1024;     // micro-cell workgroup: one thread per node, up to 1024 nodes
+constexpr int kMicroMaxBS = 1024;     // micro-cell workgroup size limit
+constexpr int kMicroNPT = 4;          // nodes per thread at most (bs >= nn / 4)
 constexpr int kMaxNodeInc = 12;       // macro elements around a node hosting a cell
 constexpr int kVecBS = 256;           // threads per workgroup of the vector kernels
 constexpr int kVecR = 4;              // rows per thread of the vector kernels
@@ -113,6 +114,8 @@ size_t      micro_lds_bytes(const DevProblem* p);
 hipError_t  launch_gfunc(const DevProblem* p, const GArgs& a, hipStream_t st);
 // one of the three G kernels: 0 macro elements, 1 micro cells, 2 macro nodes
 hipError_t  launch_gfunc_part(const DevProblem* p, const GArgs& a, int part, hipStream_t st);
+// tuned micro-cell launch shape (bs = 0 before the first G)
+void        micro_shape(const DevProblem* p, int* bs, int* wpe);
 
 // Row blocks of the vector kernels: block b covers rows [r0, r1) of scenario scen.
 struct BlkMap {

@@ -214,8 +214,13 @@ __device__ __forceinline__ bool is_dirichlet(const DTmpl& T, int i)
     }
 }
 
-template <bool STRUCT>
-__global__ __launch_bounds__(kMicroMaxBS) void k_micro(DBatch B, GArgs a)
+// Launch shape (tuned per problem by tune_micro): workgroup size bs (256..1024, a
+// multiple of 64, each thread owning nodes tid, tid + bs, ...; at most kMicroNPT) and a
+// register budget WPE (amdgpu_waves_per_eu: 1 = compiler default, 6, 8).  Results do
+// not depend on the shape: every per-node and per-element computation is the same, and
+// the exchange sum uses a fixed 256-way order.
+template <bool STRUCT, int WPE>
+__global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_micro(DBatch B, GArgs a)
 {
     extern __shared__ __align__(16) double smem[];
     __shared__ double s_bc[6];
@@ -315,11 +320,12 @@ __global__ __launch_bounds__(kMicroMaxBS) void k_micro(DBatch B, GArgs a)
     for (int e = tid; e < ne; e += bs) soil[e] = (unsigned char)elem_soil(B, T, cell, e);
     __syncthreads();
 
-    // node-centric constitutive evaluation (one node per thread: the workgroup has at
-    // least nn threads); storage kept in a register for the gather
-    double stor = 1.0;
-    {
-        const int i = tid;
+    // node-centric constitutive evaluation; storage kept in registers for the gather
+    double stor[kMicroNPT];
+#pragma unroll
+    for (int k = 0; k < kMicroNPT; ++k) {
+        stor[k] = 1.0;
+        const int i = tid + k * bs;
         if (i < nn) {
             double cva = 0.0, cvb = 0.0;
             for_incident<STRUCT>(T, i, [&](int e, int) {
@@ -345,7 +351,7 @@ __global__ __launch_bounds__(kMicroMaxBS) void k_micro(DBatch B, GArgs a)
             }
             KA[i] = kA;
             KB[i] = kB;
-            stor = cA * cva + cB * cvb;
+            stor[k] = cA * cva + cB * cvb;
         }
     }
     __syncthreads();
@@ -394,10 +400,12 @@ __global__ __launch_bounds__(kMicroMaxBS) void k_micro(DBatch B, GArgs a)
     }
     __syncthreads();
 
-    // node-centric gather (incident order = element order, identical on every run)
-    double Qacc = 0.0;
-    {
-        const int i = tid;
+    // node-centric gather (incident order = element order, identical on every run);
+    // interior net inflows go to LDS (KA is free now) for the exchange sum
+    double* netl = KA;
+#pragma unroll
+    for (int k = 0; k < kMicroNPT; ++k) {
+        const int i = tid + k * bs;
         if (i < nn) {
             double net = 0.0;
             for_incident<STRUCT>(T, i, [&](int e, int lv) {
@@ -405,19 +413,22 @@ __global__ __launch_bounds__(kMicroMaxBS) void k_micro(DBatch B, GArgs a)
                 net -= q[e * 3 + lv];
             });
             double gi;
-            if (is_dirichlet<STRUCT>(T, i)) gi = 0.0;
-            else { gi = net / stor; Qacc += net; }
+            if (is_dirichlet<STRUCT>(T, i)) { gi = 0.0; netl[i] = 0.0; }
+            else { gi = net / stor[k]; netl[i] = net; }
             store_out(B, a, s, off + i, gi);
         }
     }
-    Qacc = wave_sum(Qacc);
-    if ((tid & 63) == 0) s_red[tid >> 6] = Qacc;
     __syncthreads();
-    if (tid == 0) {
-        double Q = 0.0;
-        for (int w = 0; w < bs / 64; ++w) Q += s_red[w];
-        B.Q[cell] = Q;
+    // exchange Q = sum of interior net inflows in a fixed order (threads 0..255 take
+    // nodes t, t+256, ...; then the 4 wavefronts in order), independent of bs
+    if (tid < 256) {
+        double Qacc = 0.0;
+        for (int i = tid; i < nn; i += 256) Qacc += netl[i];
+        Qacc = wave_sum(Qacc);
+        if ((tid & 63) == 0) s_red[tid >> 6] = Qacc;
     }
+    __syncthreads();
+    if (tid == 0) B.Q[cell] = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
 }
 
 // ---------------------------------------------------------------- macro nodes
@@ -467,6 +478,10 @@ struct DevProblem {
     DBatch B{};
     size_t lds = 0;
     bool all_structured = false;      // every template is a uniform grid: analytic k_micro
+    int micro_bs = 0;                 // tuned launch shape (0 = not tuned yet)
+    int micro_wpe = 1;
+    double tune_ms[64];               // measured candidates (for reports)
+    int tune_n = 0;
 };
 
 namespace {
@@ -504,6 +519,8 @@ void csr(int nn, int ne, int nv, const int* el, std::vector<int>& ptr, std::vect
 }
 
 }  // namespace
+
+static std::vector<const void*> micro_kernels();
 
 // A template is the generator's uniform r x r right-triangle grid when its
 // connectivity, Dirichlet set and every element's geometry (to 1e-12) match the
@@ -579,7 +596,7 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
         const micro_mesh_struct& m = v.tmpl[t];
         if (m.no_verts != 3) { err = "micro templates must be triangles"; return nullptr; }
         const int nn = m.no_nodes, ne = m.no_elements;
-        if (nn > kMicroMaxBS) { err = "micro template has more than 1024 nodes"; return nullptr; }
+        if (nn > kMicroMaxBS * kMicroNPT) { err = "micro template has more than 4096 nodes"; return nullptr; }
         nn_max = std::max(nn_max, nn);
         ne_max = std::max(ne_max, ne);
         std::vector<int> ptr, inc;
@@ -751,8 +768,9 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
     P->lds = ((size_t)3 * nn_max + (size_t)3 * ne_max) * sizeof(double) + (size_t)ne_max;
     P->lds = (P->lds + 15) & ~size_t(15);
     if (P->lds > 64 * 1024) {
-        if (hipFuncSetAttribute((const void*)k_micro<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds) != hipSuccess ||
-            hipFuncSetAttribute((const void*)k_micro<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds) != hipSuccess) {
+        bool ok = true;
+        for (const void* f : micro_kernels()) ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds) == hipSuccess;
+        if (!ok) {
             err = "micro template needs more LDS than a workgroup may allocate";
             hipFree(P->buf);
             delete P;
@@ -772,6 +790,68 @@ void free_problem(DevProblem* p)
 const DBatch& batch_of(const DevProblem* p) { return p->B; }
 size_t micro_lds_bytes(const DevProblem* p) { return p->lds; }
 
+static const int kWpe[3] = {1, 6, 8};
+
+static std::vector<const void*> micro_kernels()
+{
+    return {(const void*)k_micro<false, 1>, (const void*)k_micro<false, 6>, (const void*)k_micro<false, 8>,
+            (const void*)k_micro<true, 1>, (const void*)k_micro<true, 6>, (const void*)k_micro<true, 8>};
+}
+
+static void launch_micro(const DevProblem* p, const GArgs& a, int bs, int wpe, hipStream_t st)
+{
+    const DBatch& B = p->B;
+    const dim3 g(B.n_cells), b(bs);
+    if (p->all_structured) {
+        if (wpe == 8) hipLaunchKernelGGL((k_micro<true, 8>), g, b, p->lds, st, B, a);
+        else if (wpe == 6) hipLaunchKernelGGL((k_micro<true, 6>), g, b, p->lds, st, B, a);
+        else hipLaunchKernelGGL((k_micro<true, 1>), g, b, p->lds, st, B, a);
+    } else {
+        if (wpe == 8) hipLaunchKernelGGL((k_micro<false, 8>), g, b, p->lds, st, B, a);
+        else if (wpe == 6) hipLaunchKernelGGL((k_micro<false, 6>), g, b, p->lds, st, B, a);
+        else hipLaunchKernelGGL((k_micro<false, 1>), g, b, p->lds, st, B, a);
+    }
+}
+
+// Autotuner: on the first G of a problem, time every admissible launch shape on the
+// real inputs (one warm-up and three timed launches each, HIP events) and keep the
+// fastest.  Every candidate computes the same result, so the output of the final
+// launch (the chosen shape) is the G the caller asked for.
+static hipError_t tune_micro(DevProblem* p, const GArgs& a, hipStream_t st)
+{
+    const DBatch& B = p->B;
+    const int bs_min = std::max(256, (B.nn_max + kMicroNPT - 1) / kMicroNPT);
+    std::vector<int> cand_bs;
+    for (int bs : {256, 320, 384, 448, 512, 640, 768, 1024}) {
+        const int b = (std::max(bs, bs_min) + 63) / 64 * 64;
+        if (b <= kMicroMaxBS && std::find(cand_bs.begin(), cand_bs.end(), b) == cand_bs.end()) cand_bs.push_back(b);
+    }
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return hipErrorUnknown;
+    double best = 1e300;
+    int best_bs = cand_bs[0], best_w = 1;
+    p->tune_n = 0;
+    for (int bs : cand_bs)
+        for (int w : kWpe) {
+            launch_micro(p, a, bs, w, st);
+            hipEventRecord(e0, st);
+            for (int r = 0; r < 3; ++r) launch_micro(p, a, bs, w, st);
+            hipEventRecord(e1, st);
+            if (hipEventSynchronize(e1) != hipSuccess) { hipEventDestroy(e0); hipEventDestroy(e1); return hipErrorUnknown; }
+            float ms = 0.0f;
+            hipEventElapsedTime(&ms, e0, e1);
+            const double avg = ms / 3.0;
+            if (p->tune_n < 64) p->tune_ms[p->tune_n++] = avg;
+            if (avg < best) { best = avg; best_bs = bs; best_w = w; }
+        }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    p->micro_bs = best_bs;
+    p->micro_wpe = best_w;
+    launch_micro(p, a, best_bs, best_w, st);
+    return hipGetLastError();
+}
+
 hipError_t launch_gfunc_part(const DevProblem* p, const GArgs& a, int part, hipStream_t st)
 {
     const DBatch& B = p->B;
@@ -779,16 +859,19 @@ hipError_t launch_gfunc_part(const DevProblem* p, const GArgs& a, int part, hipS
         dim3 ge((B.macro_ne_max + 255) / 256, B.n_scen);
         hipLaunchKernelGGL(k_macro_elem, ge, dim3(256), 0, st, B, a);
     } else if (part == 1) {
-        const int bs = std::min(kMicroMaxBS, std::max(64, (B.nn_max + 63) / 64 * 64));
-        if (p->all_structured)
-            hipLaunchKernelGGL(k_micro<true>, dim3(B.n_cells), dim3(bs), p->lds, st, B, a);
-        else
-            hipLaunchKernelGGL(k_micro<false>, dim3(B.n_cells), dim3(bs), p->lds, st, B, a);
+        if (p->micro_bs == 0) return tune_micro(const_cast<DevProblem*>(p), a, st);
+        launch_micro(p, a, p->micro_bs, p->micro_wpe, st);
     } else {
         dim3 gn((B.macro_nn_max + 255) / 256, B.n_scen);
         hipLaunchKernelGGL(k_macro_node, gn, dim3(256), 0, st, B, a);
     }
     return hipGetLastError();
+}
+
+void micro_shape(const DevProblem* p, int* bs, int* wpe)
+{
+    *bs = p->micro_bs;
+    *wpe = p->micro_wpe;
 }
 
 hipError_t launch_gfunc(const DevProblem* p, const GArgs& a, hipStream_t st)

@@ -810,6 +810,8 @@ int tsr_solver_gfunc_dev(tsr_solver* s, double* g_dev, const double* u_dev, int 
 
 double tsr_solver_last_gfunc_ms(const tsr_solver* s) { return s->last_g_ms; }
 
+void tsr_solver_micro_shape(const tsr_solver* s, int* bs, int* wpe) { micro_shape(s->P, bs, wpe); }
+
 int tsr_solver_time_gfunc_parts(tsr_solver* s, double* g_dev, const double* u_dev, int reps, double* ms3)
 {
     const int fail_code = 2;
