@@ -101,6 +101,17 @@ const char* tsr_last_error(void);
 
 typedef struct tsr_solver tsr_solver;
 
+/* Inputs of the learned step controller, per scenario at the start of a step:
+ *   0      log10 of the step the classical law proposes now
+ *   1..3   log10 of the last three accepted steps (newest first)
+ *   4..7   Krylov dimensions of the last four accepted steps (newest first)
+ *   8      log10 ||H_m||_inf of the last accepted step
+ *   9      log10 of its last Arnoldi residual h_{m+1,m}
+ *   10..12 mean, min, max macro head [m]
+ *   13..15 mean, min, max effective saturation of the matrix soil at the macro heads
+ * Entries without history (first steps) repeat the newest value available, or 0. */
+enum { TSR_FEAT_COUNT = 16 };
+
 /* Per-step record (the reference's statistics.txt line, main.cpp:324-329, extended with
  * what the step controller saw). */
 typedef struct tsr_step_record {
@@ -112,6 +123,9 @@ typedef struct tsr_step_record {
     double hnorm;        /* ||H_m||_inf (Hessenberg of the accepted step) */
     double resid;        /* last Arnoldi residual h_{m+1,m} */
     double head_mean, head_min, head_max, sat_mean, sat_min, sat_max;  /* macro head / saturation stats */
+    double tau_first;    /* step of the first attempt (after the Krylov-budget halvings) */
+    double err_first;    /* WRMS error estimate of the first attempt (inf if its RHS failed) */
+    float  feat[16];     /* the controller's inputs at the step start (order above) */
     int    m;            /* Krylov dimension of phi_1 */
     int    mD;           /* Krylov dimension of the error estimate */
     int    rejects;      /* rejected attempts before acceptance */
@@ -167,6 +181,46 @@ int tsr_solver_run(tsr_solver* s, double tend, int max_steps, tsr_run_stats* out
  * tsr_solver_set_mlp loads FP32 master weights (layout of tsr_mlp_param_count). */
 int tsr_solver_set_controller(tsr_solver* s, int mode);
 int tsr_solver_set_mlp(tsr_solver* s, const float* params, const float* feat_mean, const float* feat_std);
+
+/* The learned controller's network, 16 -> 64 -> 64 -> 2 with GELU; BF16 weights (round
+ * to nearest even of FP32 masters), FP32 biases, activations and accumulation.  No
+ * counterpart in the reference (its step law is src/set_exprem_options.cpp:20-26).
+ * Parameters: W1[64][16], b1[64], W2[64][64], b2[64], W3[2][64], b3[2] (FP32, row-major).
+ * Inputs of forward/grad/train are standardised features x[n][16]; targets t[n][2].
+ * Loss of grad: scale * sum_n (w0 (y_n0 - t_n0)^2 + w1 (y_n1 - t_n1)^2).  All return 0,
+ * or 2 on a device error (train: 1 on n <= 0 or batch <= 0). */
+int   tsr_mlp_param_count(void);
+void* tsr_mlp_create(void);
+void  tsr_mlp_free(void* mlp);
+int   tsr_mlp_set_params(void* mlp, const float* params);
+int   tsr_mlp_get_params(void* mlp, float* params);
+int   tsr_mlp_forward(void* mlp, const float* x, int n, float* out);
+int   tsr_mlp_grad(void* mlp, const float* x, const float* t, int n, float w0, float w1, float scale, float* grad,
+                   float* loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with the gradient of the last grad call. */
+int   tsr_mlp_adam(void* mlp, float lr);
+/* Mini-batch Adam over a seeded permutation per epoch; loss_hist[e] = mean sample loss. */
+int   tsr_mlp_train(void* mlp, const float* x, const float* t, int n, int epochs, int batch, float lr, float w0,
+                    float w1, unsigned long long seed, float* loss_hist);
+
+/* Solver path.  TSR_PATH_PLAIN (default) is the reference's EXPREM step for step: FD
+ * products of the full RHS, CGS2, the Krylov stopping rule checked on the host after
+ * every Arnoldi iteration.  The bits of the batched path:
+ *   TSR_PATH_DCGS2     Gram-Schmidt with the reorthogonalisation delayed one iteration
+ *                      (two passes over the basis per iteration instead of three)
+ *   TSR_PATH_DEFERRED  Arnoldi iterations enqueued back to back up to the predicted
+ *                      Krylov dimension; the stopping rule is then evaluated for every
+ *                      dimension of the window in order (same first passing m)
+ *   TSR_PATH_JVLIN     FD products with the micro nodes' capacities and conductivities
+ *                      expanded to first order about the step's u (the FD quotient then
+ *                      differs from the full one by O(eps) terms, as the FD quotient
+ *                      itself differs from J v)
+ * Returns 0, 1 on unknown bits, 2 on allocation failure. */
+enum { TSR_PATH_PLAIN = 0, TSR_PATH_DCGS2 = 1, TSR_PATH_DEFERRED = 2, TSR_PATH_JVLIN = 4, TSR_PATH_BATCHED = 7 };
+int tsr_solver_set_path(tsr_solver* s, int path);
+/* jv = (G(u + eps v) - G(u)) / eps about the solver's state u, with the full RHS
+ * (lin = 0) or the TSR_PATH_JVLIN expansion (lin = 1); v, jv host arrays of N. */
+int tsr_solver_jv(tsr_solver* s, const double* v, double eps, int lin, double* jv);
 
 /* Raw device pointers (for tests and the bench): u, a scratch vector of length N. */
 double* tsr_solver_dev_u(tsr_solver* s);

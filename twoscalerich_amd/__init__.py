@@ -139,6 +139,17 @@ class Solver:
         lib().tsr_solver_stats(self.h, C.byref(s))
         return s
 
+    def set_path(self, path):
+        if lib().tsr_solver_set_path(self.h, path):
+            raise RuntimeError("tsr_solver_set_path failed: " + last_error())
+
+    def jv(self, v, eps, lin=False):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.empty_like(v)
+        if lib().tsr_solver_jv(self.h, _dp(v), eps, 1 if lin else 0, _dp(out)):
+            raise RuntimeError("tsr_solver_jv failed: " + last_error())
+        return out
+
     def set_controller(self, mode):
         return lib().tsr_solver_set_controller(self.h, mode)
 

@@ -9,6 +9,7 @@ import os
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
+_fp = C.POINTER(C.c_float)
 
 
 class macro_mesh_struct(C.Structure):
@@ -72,7 +73,9 @@ class tsr_batch_view(C.Structure):
 
 class tsr_step_record(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("t", "tau", "tau_proposed", "err", "beta", "hnorm", "resid",
-                                          "head_mean", "head_min", "head_max", "sat_mean", "sat_min", "sat_max")] + \
+                                          "head_mean", "head_min", "head_max", "sat_mean", "sat_min", "sat_max",
+                                          "tau_first", "err_first")] + \
+               [("feat", C.c_float * 16)] + \
                [(n, C.c_int) for n in ("m", "mD", "rejects", "fevals", "controller", "m_cap")]
 
 
@@ -86,6 +89,7 @@ GFUNC_T = C.CFUNCTYPE(C.c_int, _dp, _dp, user_data_struct)
 
 HOST_ELEMENT, HOST_NODE = 0, 1
 CFG_PLAIN_SMALL, CFG_PLAIN, CFG_RAGGED, CFG_SCENARIOS, CFG_LARGE = 0, 1, 2, 3, 4
+PATH_PLAIN, PATH_DCGS2, PATH_DEFERRED, PATH_JVLIN, PATH_BATCHED = 0, 1, 2, 4, 7
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libtsr.so")
@@ -133,7 +137,19 @@ def declare(lib):
         "tsr_solver_run": (C.c_int, [P, C.c_double, C.c_int, C.POINTER(tsr_run_stats),
                                      C.POINTER(tsr_step_record), C.c_int]),
         "tsr_solver_set_controller": (C.c_int, [P, C.c_int]),
+        "tsr_solver_set_path": (C.c_int, [P, C.c_int]),
+        "tsr_solver_jv": (C.c_int, [P, _dp, C.c_double, C.c_int, _dp]),
         "tsr_solver_set_mlp": (C.c_int, [P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "tsr_mlp_param_count": (C.c_int, []),
+        "tsr_mlp_create": (P, []),
+        "tsr_mlp_free": (None, [P]),
+        "tsr_mlp_set_params": (C.c_int, [P, _fp]),
+        "tsr_mlp_get_params": (C.c_int, [P, _fp]),
+        "tsr_mlp_forward": (C.c_int, [P, _fp, C.c_int, _fp]),
+        "tsr_mlp_grad": (C.c_int, [P, _fp, _fp, C.c_int, C.c_float, C.c_float, C.c_float, _fp, _fp]),
+        "tsr_mlp_adam": (C.c_int, [P, C.c_float]),
+        "tsr_mlp_train": (C.c_int, [P, _fp, _fp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                    C.c_ulonglong, _fp]),
         "tsr_solver_dev_u": (C.c_void_p, [P]),
         "tsr_solver_dev_scratch": (C.c_void_p, [P, C.c_int]),
         "tsr_solver_N": (C.c_longlong, [P]),

@@ -96,6 +96,14 @@ struct DBatch {
 // Per-scenario perturbation / finite-difference arguments of one G launch:
 //   state read  = u + veps[s] * v        (v may be null)
 //   value written = (G - g0) * ieps[s]   (g0 may be null)
+//
+// Linearised Jacobian products: a G(u) launch with rec set also writes, per micro node
+// (SoA, component c at rec[c * N + global index]): 0 the head used, 1 the node's storage
+// capacity sum(scv C), 2 its derivative, 3..6 K and dK/dh of soils A and B (0 where the
+// soil is absent).  A launch with lin set (an FD product around that same u) takes the
+// micro nodes' capacities and conductivities from the first-order expansion
+// X(h) = X0 + X0' (h - h0) instead of the van Genuchten functions.
+constexpr int kLinComp = 7;
 struct GArgs {
     const double* u;
     const double* v;
@@ -104,6 +112,8 @@ struct GArgs {
     const double* ieps;
     const int* active;                // scenario mask (nullptr = all)
     double* out;
+    double* rec = nullptr;            // record the expansion point (G(u) launches)
+    const double* lin = nullptr;      // use the recorded expansion (FD launches)
 };
 
 struct DevProblem;
@@ -125,7 +135,7 @@ struct BlkMap {
 };
 
 enum PassMode { PASS_DOTS = 0, PASS_AXPY_DOTS = 1, PASS_AXPY_NORM = 2 };
-enum RedMode { RED_H1 = 0, RED_H2 = 1, RED_NORM = 2, RED_START = 3, RED_SUM = 4 };
+enum RedMode { RED_H1 = 0, RED_H2 = 1, RED_NORM = 2, RED_START = 3, RED_SUM = 4, RED_COLS = 5 };
 
 struct KrylovState {              // per-scenario device arrays, stride ld
     double* cs;                   // column scales: v_k = cs[k] * raw column k
@@ -135,7 +145,11 @@ struct KrylovState {              // per-scenario device arrays, stride ld
     double* veps;                 // eps * cs[j] for the next G call
     double* ieps;                 // 1 / eps
     double* res;                  // misc results per scenario (8 slots)
-    int ld, ldh, hstride;
+    double* coefr;                // DCGS2: coefficients of the delayed correction of column j
+    double* dots;                 // DCGS2: reduced sums of k_pass_dca, stride ld2
+    double* nwj;                  // ||w||^2 before projection, per iteration j (stride ld)
+    double* wrj;                  // WRMS of the new column, per iteration j (stride ld)
+    int ld, ldh, hstride, ld2;
 };
 
 hipError_t launch_pass(int mode, const double* const* cols, int nk, double* w, const double* u, const double* coef,
@@ -149,8 +163,16 @@ hipError_t launch_lincomb(double* out, const double* base, const double* const* 
                           const int* active, hipStream_t st);
 hipError_t launch_reduce(int mode, const double* part, int ldp, int nk, const int* blk_first, int n_scen,
                          const long long* scen_rows, KrylovState K, int j, const int* active, hipStream_t st);
+// One Arnoldi orthogonalisation by DCGS2 (nk = j + 1 columns; w becomes raw column j+1
+// and column j is corrected in place): k_pass_dca, column reduction, k_dc_finish,
+// k_pass_dcb, then the RED_NORM bookkeeping.
+hipError_t launch_dcgs2(int nk, const double* const* cols, double* w, const double* u, KrylovState K, const BlkMap& M,
+                       int nblk, double* part, int ldp, const int* blk_first, int n_scen, const long long* scen_rows,
+                       const int* active, double at, double rt, hipStream_t st);
 hipError_t launch_masked_copy(double* dst, const double* src, const BlkMap& M, int nblk, const int* mask,
                               hipStream_t st);
+// out[s*6 + 0..5]: mean, min, max macro head; mean, min, max matrix-soil saturation
+hipError_t launch_macro_stats(const DScen* scen, int n_scen, const double* u, double* out, hipStream_t st);
 
 }  // namespace tsr
 

@@ -48,6 +48,36 @@ __device__ __forceinline__ void vg_ck(const DSoil& s, double h, double Ss, doubl
     }
 }
 
+// vg_ck plus dC/dh and dK/dh (same C and K, bit for bit).  With x = (alpha|h|)^n,
+// Se = (1+x)^-m, B = 1 - (x/(1+x))^m, K = Ks sqrt(Se) B^2, C - Ss = Cw:
+//   dC/dh = Cw / h * (n - 1 - n (1+m) x / (1+x))
+//   dK/dh = -K m n / ((1+x) h) * (x/2 + 2 (1-B)/B)
+// Non-finite derivatives (h -> 0-) are recorded as 0.
+__device__ __forceinline__ void vg_ckd(const DSoil& s, double h, double Ss, double& C, double& K, double& dC,
+                                       double& dK)
+{
+    if (h < 0.0) {
+        const double L = log(s.al * (-h));
+        const double x = exp(s.n * L);
+        const double P = log1p(x);
+        const double Se = exp(-s.m * P);
+        const double Cw = (s.ts - s.tr) * s.m * s.n * x * Se / ((-h) * (1.0 + x));
+        C = Cw + Ss;
+        const double B = -expm1(-s.m * log1p(1.0 / x));
+        K = s.Ks * sqrt(Se) * B * B;
+        const double ih = 1.0 / h, ix1 = 1.0 / (1.0 + x);
+        dC = Cw * ih * (s.n - 1.0 - s.n * (1.0 + s.m) * x * ix1);
+        dK = -K * s.m * s.n * ix1 * ih * (0.5 * x + 2.0 * (1.0 - B) / B);
+        if (!isfinite(dC)) dC = 0.0;
+        if (!isfinite(dK)) dK = 0.0;
+    } else {
+        C = Ss;
+        K = s.Ks;
+        dC = 0.0;
+        dK = 0.0;
+    }
+}
+
 __device__ __forceinline__ void keff_dev(const DScen& S, double h, double K[4])
 {
     const int n = S.keff_n;
@@ -219,7 +249,10 @@ __device__ __forceinline__ bool is_dirichlet(const DTmpl& T, int i)
 // register budget WPE (amdgpu_waves_per_eu: 1 = compiler default, 6, 8).  Results do
 // not depend on the shape: every per-node and per-element computation is the same, and
 // the exchange sum uses a fixed 256-way order.
-template <bool STRUCT, int WPE>
+//
+// MODE 0 evaluates G; MODE 1 also records the expansion point (GArgs::rec); MODE 2 takes
+// capacities and conductivities from the recorded first-order expansion (GArgs::lin).
+template <bool STRUCT, int WPE, int MODE>
 __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_micro(DBatch B, GArgs a)
 {
     extern __shared__ __align__(16) double smem[];
@@ -322,17 +355,28 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
 
     // node-centric constitutive evaluation; storage kept in registers for the gather
     double stor[kMicroNPT];
+    const size_t NN = (size_t)B.N;
 #pragma unroll
     for (int k = 0; k < kMicroNPT; ++k) {
         stor[k] = 1.0;
         const int i = tid + k * bs;
+        if constexpr (MODE == 2) {
+            if (i < nn) {
+                const double* L = a.lin + off + i;
+                const double d = h[i] - L[0];
+                stor[k] = L[NN] + L[2 * NN] * d;
+                KA[i] = L[3 * NN] + L[4 * NN] * d;
+                KB[i] = L[5 * NN] + L[6 * NN] * d;
+            }
+            continue;
+        }
         if (i < nn) {
             double cva = 0.0, cvb = 0.0;
             for_incident<STRUCT>(T, i, [&](int e, int) {
                 const double sc = STRUCT ? s_geo[e & 1][21] : T.scv[e];
                 if (soil[e] == TSR_SOIL_A) cva += sc; else cvb += sc;
             });
-            double cA = 0.0, cB = 0.0, kA = 0.0, kB = 0.0;
+            double cA = 0.0, cB = 0.0, kA = 0.0, kB = 0.0, dcA = 0.0, dcB = 0.0, dkA = 0.0, dkB = 0.0;
             // first evaluation with the lane's own soil (no divergence), second only at interfaces
             const bool hasA = cva > 0.0, hasB = cvb > 0.0;
             {
@@ -340,18 +384,30 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
                 sl.Ks = hasA ? soilA.Ks : soilB.Ks; sl.tr = hasA ? soilA.tr : soilB.tr;
                 sl.ts = hasA ? soilA.ts : soilB.ts; sl.al = hasA ? soilA.al : soilB.al;
                 sl.n = hasA ? soilA.n : soilB.n; sl.m = hasA ? soilA.m : soilB.m;
-                double C, K;
-                vg_ck(sl, h[i], B.Ss, C, K);
-                if (hasA) { cA = C; kA = K; } else { cB = C; kB = K; }
+                double C, K, dC = 0.0, dK = 0.0;
+                if constexpr (MODE == 1) vg_ckd(sl, h[i], B.Ss, C, K, dC, dK);
+                else vg_ck(sl, h[i], B.Ss, C, K);
+                if (hasA) { cA = C; kA = K; dcA = dC; dkA = dK; } else { cB = C; kB = K; dcB = dC; dkB = dK; }
             }
             if (hasA && hasB) {
-                double C, K;
-                vg_ck(soilB, h[i], B.Ss, C, K);
-                cB = C; kB = K;
+                double C, K, dC = 0.0, dK = 0.0;
+                if constexpr (MODE == 1) vg_ckd(soilB, h[i], B.Ss, C, K, dC, dK);
+                else vg_ck(soilB, h[i], B.Ss, C, K);
+                cB = C; kB = K; dcB = dC; dkB = dK;
             }
             KA[i] = kA;
             KB[i] = kB;
             stor[k] = cA * cva + cB * cvb;
+            if constexpr (MODE == 1) {
+                double* R = a.rec + off + i;
+                R[0] = h[i];
+                R[NN] = stor[k];
+                R[2 * NN] = dcA * cva + dcB * cvb;
+                R[3 * NN] = kA;
+                R[4 * NN] = dkA;
+                R[5 * NN] = kB;
+                R[6 * NN] = dkB;
+            }
         }
     }
     __syncthreads();
@@ -478,8 +534,8 @@ struct DevProblem {
     DBatch B{};
     size_t lds = 0;
     bool all_structured = false;      // every template is a uniform grid: analytic k_micro
-    int micro_bs = 0;                 // tuned launch shape (0 = not tuned yet)
-    int micro_wpe = 1;
+    int micro_bs[3] = {0, 0, 0};      // tuned launch shape per k_micro mode (0 = not tuned yet)
+    int micro_wpe[3] = {1, 1, 1};
     double tune_ms[64];               // measured candidates (for reports)
     int tune_n = 0;
 };
@@ -792,34 +848,46 @@ size_t micro_lds_bytes(const DevProblem* p) { return p->lds; }
 
 static const int kWpe[3] = {1, 6, 8};
 
+using MicroFn = void (*)(DBatch, GArgs);
+
+template <bool ST, int M>
+static MicroFn micro_fn(int wpe)
+{
+    return wpe == 8 ? (MicroFn)k_micro<ST, 8, M> : wpe == 6 ? (MicroFn)k_micro<ST, 6, M> : (MicroFn)k_micro<ST, 1, M>;
+}
+
+static MicroFn micro_fn(bool structured, int mode, int wpe)
+{
+    if (structured) return mode == 2 ? micro_fn<true, 2>(wpe) : mode == 1 ? micro_fn<true, 1>(wpe) : micro_fn<true, 0>(wpe);
+    return mode == 2 ? micro_fn<false, 2>(wpe) : mode == 1 ? micro_fn<false, 1>(wpe) : micro_fn<false, 0>(wpe);
+}
+
 static std::vector<const void*> micro_kernels()
 {
-    return {(const void*)k_micro<false, 1>, (const void*)k_micro<false, 6>, (const void*)k_micro<false, 8>,
-            (const void*)k_micro<true, 1>, (const void*)k_micro<true, 6>, (const void*)k_micro<true, 8>};
+    std::vector<const void*> v;
+    for (int st = 0; st < 2; ++st)
+        for (int mode = 0; mode < 3; ++mode)
+            for (int w : {1, 6, 8}) v.push_back((const void*)micro_fn(st == 1, mode, w));
+    return v;
 }
+
+static int micro_mode(const GArgs& a) { return a.lin ? 2 : a.rec ? 1 : 0; }
 
 static void launch_micro(const DevProblem* p, const GArgs& a, int bs, int wpe, hipStream_t st)
 {
     const DBatch& B = p->B;
-    const dim3 g(B.n_cells), b(bs);
-    if (p->all_structured) {
-        if (wpe == 8) hipLaunchKernelGGL((k_micro<true, 8>), g, b, p->lds, st, B, a);
-        else if (wpe == 6) hipLaunchKernelGGL((k_micro<true, 6>), g, b, p->lds, st, B, a);
-        else hipLaunchKernelGGL((k_micro<true, 1>), g, b, p->lds, st, B, a);
-    } else {
-        if (wpe == 8) hipLaunchKernelGGL((k_micro<false, 8>), g, b, p->lds, st, B, a);
-        else if (wpe == 6) hipLaunchKernelGGL((k_micro<false, 6>), g, b, p->lds, st, B, a);
-        else hipLaunchKernelGGL((k_micro<false, 1>), g, b, p->lds, st, B, a);
-    }
+    MicroFn f = micro_fn(p->all_structured, micro_mode(a), wpe);
+    hipLaunchKernelGGL(f, dim3(B.n_cells), dim3(bs), p->lds, st, B, a);
 }
 
-// Autotuner: on the first G of a problem, time every admissible launch shape on the
-// real inputs (one warm-up and three timed launches each, HIP events) and keep the
-// fastest.  Every candidate computes the same result, so the output of the final
-// launch (the chosen shape) is the G the caller asked for.
+// Autotuner: on the first G of a problem (per k_micro mode), time every admissible
+// launch shape on the real inputs (one warm-up and three timed launches each, HIP
+// events) and keep the fastest.  Every candidate computes the same result, so the
+// output of the final launch (the chosen shape) is the G the caller asked for.
 static hipError_t tune_micro(DevProblem* p, const GArgs& a, hipStream_t st)
 {
     const DBatch& B = p->B;
+    const int mode = micro_mode(a);
     const int bs_min = std::max(256, (B.nn_max + kMicroNPT - 1) / kMicroNPT);
     std::vector<int> cand_bs;
     for (int bs : {256, 320, 384, 448, 512, 640, 768, 1024}) {
@@ -830,7 +898,7 @@ static hipError_t tune_micro(DevProblem* p, const GArgs& a, hipStream_t st)
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return hipErrorUnknown;
     double best = 1e300;
     int best_bs = cand_bs[0], best_w = 1;
-    p->tune_n = 0;
+    if (mode == 0) p->tune_n = 0;
     for (int bs : cand_bs)
         for (int w : kWpe) {
             launch_micro(p, a, bs, w, st);
@@ -841,13 +909,13 @@ static hipError_t tune_micro(DevProblem* p, const GArgs& a, hipStream_t st)
             float ms = 0.0f;
             hipEventElapsedTime(&ms, e0, e1);
             const double avg = ms / 3.0;
-            if (p->tune_n < 64) p->tune_ms[p->tune_n++] = avg;
+            if (mode == 0 && p->tune_n < 64) p->tune_ms[p->tune_n++] = avg;
             if (avg < best) { best = avg; best_bs = bs; best_w = w; }
         }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    p->micro_bs = best_bs;
-    p->micro_wpe = best_w;
+    p->micro_bs[mode] = best_bs;
+    p->micro_wpe[mode] = best_w;
     launch_micro(p, a, best_bs, best_w, st);
     return hipGetLastError();
 }
@@ -859,8 +927,9 @@ hipError_t launch_gfunc_part(const DevProblem* p, const GArgs& a, int part, hipS
         dim3 ge((B.macro_ne_max + 255) / 256, B.n_scen);
         hipLaunchKernelGGL(k_macro_elem, ge, dim3(256), 0, st, B, a);
     } else if (part == 1) {
-        if (p->micro_bs == 0) return tune_micro(const_cast<DevProblem*>(p), a, st);
-        launch_micro(p, a, p->micro_bs, p->micro_wpe, st);
+        const int mode = micro_mode(a);
+        if (p->micro_bs[mode] == 0) return tune_micro(const_cast<DevProblem*>(p), a, st);
+        launch_micro(p, a, p->micro_bs[mode], p->micro_wpe[mode], st);
     } else {
         dim3 gn((B.macro_nn_max + 255) / 256, B.n_scen);
         hipLaunchKernelGGL(k_macro_node, gn, dim3(256), 0, st, B, a);
@@ -870,8 +939,8 @@ hipError_t launch_gfunc_part(const DevProblem* p, const GArgs& a, int part, hipS
 
 void micro_shape(const DevProblem* p, int* bs, int* wpe)
 {
-    *bs = p->micro_bs;
-    *wpe = p->micro_wpe;
+    *bs = p->micro_bs[0];
+    *wpe = p->micro_wpe[0];
 }
 
 hipError_t launch_gfunc(const DevProblem* p, const GArgs& a, hipStream_t st)

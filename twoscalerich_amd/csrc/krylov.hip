@@ -140,6 +140,189 @@ __global__ __launch_bounds__(kVecBS) void k_pass(int mode, const double* const* 
     }
 }
 
+// ---------------------------------------------------------------- DCGS2
+// Classical Gram-Schmidt with the reorthogonalisation of each column delayed by one
+// iteration (the "DCGS2" scheme of Swirydowicz, Langou, Ananthan, Yang and Thomas,
+// Numer. Linear Algebra Appl. 2021), so one Arnoldi iteration reads the basis twice
+// instead of three times.  At iteration j the raw columns R_0..R_{j-1} are final and
+// R_j is the once-projected previous vector with provisional scale cs_j.
+//
+// k_pass_dca, per row (nk = j + 1 columns):
+//   part[k][b]        = sum R_k w      (k <= j)
+//   part[nk + k][b]   = sum R_k R_j    (k < j)
+//   part[2 nk - 1][b] = sum R_j R_j
+//   part[2 nk][b]     = sum w w
+__global__ __launch_bounds__(kVecBS) void k_pass_dca(const double* const* cols, int nk, const double* w, BlkMap M,
+                                                    double* part, int ldp, const int* active)
+{
+    __shared__ double lds[kVecBS / 64][2 * kMaxCols + 2];
+    const int b = blockIdx.x;
+    const int s = M.scen[b];
+    if (active && !active[s]) return;
+    const long long r0 = M.r0[b], r1 = M.r1[b];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int j = nk - 1;
+    double wr[kVecR], rj[kVecR];
+    long long idx[kVecR];
+    bool ok[kVecR];
+    const double* Rj = cols[j];
+#pragma unroll
+    for (int r = 0; r < kVecR; ++r) {
+        idx[r] = r0 + tid + (long long)r * kVecBS;
+        ok[r] = idx[r] < r1;
+        wr[r] = ok[r] ? w[idx[r]] : 0.0;
+        rj[r] = ok[r] ? Rj[idx[r]] : 0.0;
+    }
+    int k = 0;
+    for (; k + 2 <= j; k += 2) {
+        const double* c0 = cols[k];
+        const double* c1 = cols[k + 1];
+        double s0 = 0.0, s1 = 0.0, p0 = 0.0, p1 = 0.0;
+#pragma unroll
+        for (int r = 0; r < kVecR; ++r)
+            if (ok[r]) {
+                const double a0 = c0[idx[r]], a1 = c1[idx[r]];
+                s0 += a0 * wr[r]; s1 += a1 * wr[r];
+                p0 += a0 * rj[r]; p1 += a1 * rj[r];
+            }
+        s0 = wave_sum_v(s0); s1 = wave_sum_v(s1); p0 = wave_sum_v(p0); p1 = wave_sum_v(p1);
+        if (lane == 0) { lds[wv][k] = s0; lds[wv][k + 1] = s1; lds[wv][nk + k] = p0; lds[wv][nk + k + 1] = p1; }
+    }
+    for (; k < j; ++k) {
+        const double* c0 = cols[k];
+        double s0 = 0.0, p0 = 0.0;
+#pragma unroll
+        for (int r = 0; r < kVecR; ++r)
+            if (ok[r]) { const double a0 = c0[idx[r]]; s0 += a0 * wr[r]; p0 += a0 * rj[r]; }
+        s0 = wave_sum_v(s0); p0 = wave_sum_v(p0);
+        if (lane == 0) { lds[wv][k] = s0; lds[wv][nk + k] = p0; }
+    }
+    {
+        double sj = 0.0, rr = 0.0, ww = 0.0;
+#pragma unroll
+        for (int r = 0; r < kVecR; ++r) { sj += rj[r] * wr[r]; rr += rj[r] * rj[r]; ww += wr[r] * wr[r]; }
+        sj = wave_sum_v(sj); rr = wave_sum_v(rr); ww = wave_sum_v(ww);
+        if (lane == 0) { lds[wv][j] = sj; lds[wv][2 * nk - 1] = rr; lds[wv][2 * nk] = ww; }
+    }
+    __syncthreads();
+    for (int kk = tid; kk <= 2 * nk; kk += kVecBS) {
+        double acc = 0.0;
+        for (int x = 0; x < kVecBS / 64; ++x) acc += lds[x][kk];
+        part[(size_t)kk * ldp + b] = acc;
+    }
+}
+
+// Coefficients of iteration j from the reduced sums d (layout of k_pass_dca), one
+// thread per scenario.  With a_k = <v_k, v~_j> (v~_j = cs_j R_j) and
+// gamma = ||v~_j - sum a_k v_k||:
+//   v_j = (v~_j - sum_k a_k v_k) / gamma          (R_j -= sum coefr_k R_k, cs_j /= gamma)
+//   column j-1 of H: H[k][j-1] += a_k H[j][j-1], H[j][j-1] *= gamma
+//   column j:  h_k = cs_k d_k (k < j), h_j = (cs_j d_j - sum a_k h_k) / gamma
+//   w' = w - sum_{k<j} (h_k - h_j a_k / gamma) cs_k R_k - (h_j cs_j / gamma) R_j
+// res[1] and nwj[j] receive ||w||^2 (before projection).
+__global__ void k_dc_finish(KrylovState K, int j, int n_scen, const int* active)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_scen || (active && !active[s])) return;
+    const int nk = j + 1;
+    const double* d = K.dots + (size_t)s * K.ld2;
+    double* cs = K.cs + (size_t)s * K.ld;
+    double* coef = K.coef + (size_t)s * K.ld;
+    double* coefr = K.coefr + (size_t)s * K.ld;
+    double* H = K.H + (size_t)s * K.hstride;
+    double* hj_col = H + (size_t)j * K.ldh;
+    K.res[(size_t)s * 8 + 1] = d[2 * nk];
+    K.nwj[(size_t)s * K.ld + j] = d[2 * nk];
+    if (j == 0) {
+        const double h0 = cs[0] * d[0];
+        hj_col[0] = h0;
+        coef[0] = cs[0] * h0;
+        return;
+    }
+    const double csj = cs[j];
+    double suma2 = 0.0;
+    for (int k = 0; k < j; ++k) {
+        const double a = cs[k] * csj * d[nk + k];
+        coefr[k] = a;   // a_k for now
+        suma2 += a * a;
+    }
+    const double g2 = csj * csj * d[2 * nk - 1] - suma2;
+    const double gam = (csj > 0.0 && g2 > 0.0) ? sqrt(g2) : 1.0;
+    double* hp = H + (size_t)(j - 1) * K.ldh;
+    const double hprev = hp[j];
+    for (int k = 0; k < j; ++k) hp[k] += coefr[k] * hprev;
+    hp[j] = hprev * gam;
+    double ah = 0.0;
+    for (int k = 0; k < j; ++k) {
+        const double hk = cs[k] * d[k];
+        hj_col[k] = hk;
+        ah += coefr[k] * hk;
+    }
+    const double hjj = (csj * d[j] - ah) / gam;
+    hj_col[j] = hjj;
+    for (int k = 0; k < j; ++k) {
+        const double a = coefr[k];
+        coef[k] = (hj_col[k] - hjj * a / gam) * cs[k];
+        coefr[k] = csj > 0.0 ? a * cs[k] / csj : 0.0;
+    }
+    coef[j] = hjj * csj / gam;
+    cs[j] = csj / gam;
+}
+
+// w -= sum_{k<=j} coef_k R_k;  R_j -= sum_{k<j} coefr_k R_k (both from the old R_j);
+// part[0][b] = sum w'^2, part[1][b] = sum (w'/wt(u))^2.
+__global__ __launch_bounds__(kVecBS) void k_pass_dcb(const double* const* cols, int nk, double* w, const double* u,
+                                                    const double* coef, const double* coefr, int ldc, BlkMap M,
+                                                    double* part, int ldp, const int* active, double at, double rt)
+{
+    __shared__ double red[kVecBS / 64];
+    const int b = blockIdx.x;
+    const int s = M.scen[b];
+    if (active && !active[s]) return;
+    const long long r0 = M.r0[b], r1 = M.r1[b];
+    const int tid = threadIdx.x;
+    const int j = nk - 1;
+    const double* cf = coef + (size_t)s * ldc;
+    const double* cr = coefr + (size_t)s * ldc;
+    double* Rj = const_cast<double*>(cols[j]);
+    double wr[kVecR], rj[kVecR], rn[kVecR];
+    long long idx[kVecR];
+    bool ok[kVecR];
+    const double cj = cf[j];
+#pragma unroll
+    for (int r = 0; r < kVecR; ++r) {
+        idx[r] = r0 + tid + (long long)r * kVecBS;
+        ok[r] = idx[r] < r1;
+        rj[r] = ok[r] ? Rj[idx[r]] : 0.0;
+        wr[r] = ok[r] ? w[idx[r]] - cj * rj[r] : 0.0;
+        rn[r] = rj[r];
+    }
+    for (int k = 0; k < j; ++k) {
+        const double c = cf[k], c2 = cr[k];
+        const double* col = cols[k];
+#pragma unroll
+        for (int r = 0; r < kVecR; ++r)
+            if (ok[r]) {
+                const double a = col[idx[r]];
+                wr[r] -= a * c;
+                rn[r] -= a * c2;
+            }
+    }
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int r = 0; r < kVecR; ++r)
+        if (ok[r]) {
+            w[idx[r]] = wr[r];
+            if (j > 0) Rj[idx[r]] = rn[r];
+            s0 += wr[r] * wr[r];
+            const double q = wr[r] / (at + rt * fabs(u[idx[r]]));
+            s1 += q * q;
+        }
+    s0 = block_sum(s0, red);
+    s1 = block_sum(s1, red);
+    if (tid == 0) { part[b] = s0; part[(size_t)ldp + b] = s1; }
+}
+
 // part[0..2][b] = sum a^2, sum (a/wt(u))^2, sum u^2
 __global__ __launch_bounds__(kVecBS) void k_norms(const double* a, const double* u, BlkMap M, double* part, int ldp,
                                                  const int* active, double at, double rt)
@@ -263,9 +446,13 @@ __global__ __launch_bounds__(256) void k_reduce(int mode, const double* part, in
     double* coef = K.coef + (size_t)s * K.ld;
     double* H = K.H + (size_t)s * K.hstride;
     double* res = K.res + (size_t)s * 8;
+    if (mode == RED_COLS) {
+        K.dots[(size_t)s * K.ld2 + k] = acc;
+        return;
+    }
     if (mode == RED_H1 || mode == RED_H2) {
         if (k == nk) {
-            if (mode == RED_H1) res[1] = acc;
+            if (mode == RED_H1) { res[1] = acc; K.nwj[(size_t)s * K.ld + j] = acc; }
             return;
         }
         const double hk = cs[k] * acc;
@@ -274,13 +461,16 @@ __global__ __launch_bounds__(256) void k_reduce(int mode, const double* part, in
         coef[k] = cs[k] * hk;
     } else if (mode == RED_NORM) {
         if (k == 0) {
+            // breakdown (the host's test): the next column is zero and so is its FD step
             const double n = sqrt(acc);
+            const bool bd = !(n > 1e-12 * sqrt(K.nwj[(size_t)s * K.ld + j]));
             H[(size_t)j * K.ldh + j + 1] = n;
-            cs[j + 1] = 1.0 / n;
+            cs[j + 1] = bd ? 0.0 : 1.0 / n;
             K.veps[s] = K.eps[s] * cs[j + 1];
             res[0] = n;
         } else if (k == 1) {
             res[2] = sqrt(acc / (double)scen_rows[s]);
+            K.wrj[(size_t)s * K.ld + j] = res[2];
         }
     } else if (mode == RED_START) {
         if (k == 0) {
@@ -307,7 +497,57 @@ __global__ void k_start_finish(KrylovState K, int n_scen, const int* active)
     K.veps[s] = K.eps[s] * K.cs[(size_t)s * K.ld];
 }
 
+// Per-scenario state summary for the step controller: out[s*6 + 0..5] = mean, min, max of
+// the macro heads and of the matrix soil's effective saturation
+// Se(h) = (1 + (alpha |h|)^n)^(-m) (1 for h >= 0).  One workgroup per scenario; the
+// sums run in a fixed order (thread-strided, then the wavefront tree, then the four
+// wavefronts in order).
+__global__ __launch_bounds__(256) void k_macro_stats(const DScen* scen, const double* u, double* out)
+{
+    __shared__ double red[6][4];
+    const int s = blockIdx.x, tid = threadIdx.x;
+    const DScen& S = scen[s];
+    const double* h = u + S.u_off;
+    double sh = 0.0, mnh = INFINITY, mxh = -INFINITY, ss = 0.0, mns = INFINITY, mxs = -INFINITY;
+    for (int j = tid; j < S.nn; j += 256) {
+        const double hj = h[j];
+        const double se = hj < 0.0 ? exp(-S.A.m * log1p(pow(S.A.al * (-hj), S.A.n))) : 1.0;
+        sh += hj; mnh = fmin(mnh, hj); mxh = fmax(mxh, hj);
+        ss += se; mns = fmin(mns, se); mxs = fmax(mxs, se);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        sh += __shfl_xor(sh, o, 64);
+        ss += __shfl_xor(ss, o, 64);
+        mnh = fmin(mnh, __shfl_xor(mnh, o, 64));
+        mxh = fmax(mxh, __shfl_xor(mxh, o, 64));
+        mns = fmin(mns, __shfl_xor(mns, o, 64));
+        mxs = fmax(mxs, __shfl_xor(mxs, o, 64));
+    }
+    if ((tid & 63) == 0) {
+        const int w = tid >> 6;
+        red[0][w] = sh; red[1][w] = mnh; red[2][w] = mxh; red[3][w] = ss; red[4][w] = mns; red[5][w] = mxs;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double* o = out + (size_t)s * 6;
+        const double n = (double)S.nn;
+        o[0] = (((red[0][0] + red[0][1]) + red[0][2]) + red[0][3]) / n;
+        o[1] = fmin(fmin(red[1][0], red[1][1]), fmin(red[1][2], red[1][3]));
+        o[2] = fmax(fmax(red[2][0], red[2][1]), fmax(red[2][2], red[2][3]));
+        o[3] = (((red[3][0] + red[3][1]) + red[3][2]) + red[3][3]) / n;
+        o[4] = fmin(fmin(red[4][0], red[4][1]), fmin(red[4][2], red[4][3]));
+        o[5] = fmax(fmax(red[5][0], red[5][1]), fmax(red[5][2], red[5][3]));
+    }
+}
+
 // ---------------------------------------------------------------- launchers
+hipError_t launch_macro_stats(const DScen* scen, int n_scen, const double* u, double* out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_macro_stats, dim3(n_scen), dim3(256), 0, st, scen, u, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_pass(int mode, const double* const* cols, int nk, double* w, const double* u, const double* coef,
                        int ldc, const BlkMap& M, int nblk, double* part, int ldp, const int* active, double at,
                        double rt, hipStream_t st)
@@ -342,6 +582,22 @@ hipError_t launch_reduce(int mode, const double* part, int ldp, int nk, const in
                        active);
     if (mode == RED_START)
         hipLaunchKernelGGL(k_start_finish, dim3((n_scen + 255) / 256), dim3(256), 0, st, K, n_scen, active);
+    return hipGetLastError();
+}
+
+hipError_t launch_dcgs2(int nk, const double* const* cols, double* w, const double* u, KrylovState K, const BlkMap& M,
+                       int nblk, double* part, int ldp, const int* blk_first, int n_scen, const long long* scen_rows,
+                       const int* active, double at, double rt, hipStream_t st)
+{
+    const int j = nk - 1;
+    hipLaunchKernelGGL(k_pass_dca, dim3(nblk), dim3(kVecBS), 0, st, cols, nk, w, M, part, ldp, active);
+    hipLaunchKernelGGL(k_reduce, dim3(n_scen, 2 * nk + 1), dim3(256), 0, st, (int)RED_COLS, part, ldp, 2 * nk + 1,
+                       blk_first, scen_rows, K, j, active);
+    hipLaunchKernelGGL(k_dc_finish, dim3((n_scen + 63) / 64), dim3(64), 0, st, K, j, n_scen, active);
+    hipLaunchKernelGGL(k_pass_dcb, dim3(nblk), dim3(kVecBS), 0, st, cols, nk, w, u, K.coef, K.coefr, K.ld, M, part, ldp,
+                       active, at, rt);
+    hipLaunchKernelGGL(k_reduce, dim3(n_scen, 2), dim3(256), 0, st, (int)RED_NORM, part, ldp, 2, blk_first, scen_rows, K,
+                       j, active);
     return hipGetLastError();
 }
 

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -63,9 +64,16 @@ struct ScState {
     double phi_tau = -1.0;    // (tau, m) of the phi_1 matrix currently in phi
     int phi_m = -1;
     exprem_stats_struct stats{};
-    // recent history for the learned controller
+    // recent history and state summary for the step controller
     double hist_tau[4] = {0, 0, 0, 0};
     double hist_m[4] = {0, 0, 0, 0};
+    int    n_hist = 0;            // accepted steps seen (saturates at 4)
+    double last_hnorm = 0.0, last_resid = 0.0;
+    double mstat[6] = {0, 0, 0, 0, 0, 0};
+    float  feat[16] = {};
+    double tau_first = 0.0, err_first = 0.0;
+    bool   first_done = false;
+    int    pred_m = 0, pred_mD = 0;   // predicted Krylov dimensions (deferred checks)
 };
 
 struct tsr_solver {
@@ -104,13 +112,19 @@ struct tsr_solver {
     // controller
     int ctrl_mode = 0;
     Mlp* mlp = nullptr;
-    std::vector<double> macro_h;
     // timing of the last run
     double g_ms_acc = 0.0;
     int    g_calls = 0;
     double* pin_h = nullptr;  // pinned host buffers for per-iteration readbacks
     double* pin_res = nullptr;
     int* pin_bad = nullptr;
+    double* mstats = nullptr;   // device [S*6] macro head / saturation summary
+    double* pin_ms = nullptr;
+    double* pin_cs = nullptr;   // per-window readbacks: column scales, ||w||^2, WRMS per iteration
+    double* pin_nw = nullptr;
+    double* pin_wr = nullptr;
+    int path = TSR_PATH_PLAIN;
+    double* linrec = nullptr;   // TSR_PATH_JVLIN: expansion point of the FD products (kLinComp x N)
     double t_dense = 0.0;     // host seconds in the small dense phi routines
     double t_sync = 0.0;      // host seconds waiting for the stream
 };
@@ -175,6 +189,10 @@ int G_call(tsr_solver* s, const double* v, const double* veps, const double* g0,
     }
     GArgs a;
     a.u = s->u; a.v = v; a.veps = veps; a.g0 = g0; a.ieps = ieps; a.active = active; a.out = out;
+    if (s->path & TSR_PATH_JVLIN) {
+        if (!v && out == s->g0) a.rec = s->linrec;          // G(u) at the step start
+        else if (v && g0 == s->g0) a.lin = s->linrec;      // FD product around that u
+    }
     HIPCHK(launch_gfunc(s->P, a, s->st));
     s->g_calls++;
     return 0;
@@ -237,98 +255,134 @@ double krylov_est_main(ScState& c, int m, double tau, int ldh, int ldphi)
 }
 
 // Arnoldi over the active scenarios in s->mask_h.  basis 0: main (V, KV, H), 1: error (W, KW, HD).
-// Returns 0, -1 if an RHS evaluation failed (per scenario: failed flag), 2 on device error.
+//
+// The host checks the stopping rule at every Krylov dimension m = 1, 2, ... in order and
+// stops at the first m that passes (or at the scenario's cap), as the reference's rule
+// does.  On the plain path it syncs after every iteration.  With TSR_PATH_DEFERRED the
+// iterations up to a predicted dimension (learned, or the last step's, minus two) are
+// enqueued back to back and the rule is then evaluated for every m of that window in
+// order; iterations beyond the first passing m are wasted work, not a different result.
+// Returns 0, or 2 on device error (an RHS failure sets the scenario's failed flag).
+int read_window(tsr_solver* s, KrylovState& K, int c0, int jt)
+{
+    const int fail_code = 2;
+    const int S = s->S;
+    const int ncol = jt - c0;
+    HIPCHK(hipMemcpy2DAsync(s->pin_h, sizeof(double) * ncol * K.ldh, K.H + (size_t)c0 * K.ldh,
+                            sizeof(double) * K.hstride, sizeof(double) * ncol * K.ldh, S, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipMemcpy2DAsync(s->pin_cs, sizeof(double) * (jt + 1), K.cs, sizeof(double) * K.ld,
+                            sizeof(double) * (jt + 1), S, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipMemcpy2DAsync(s->pin_nw, sizeof(double) * jt, K.nwj, sizeof(double) * K.ld, sizeof(double) * jt, S,
+                            hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipMemcpy2DAsync(s->pin_wr, sizeof(double) * jt, K.wrj, sizeof(double) * K.ld, sizeof(double) * jt, S,
+                            hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipMemcpyAsync(s->pin_bad, s->bad, sizeof(int) * S, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(timed_sync(s));
+    std::memcpy(s->bad_h.data(), s->pin_bad, sizeof(int) * S);
+    return 0;
+}
+
 int arnoldi(tsr_solver* s, int which)
 {
     const int fail_code = 2;
+    const int S = s->S;
     std::vector<double*>& B = which == 0 ? s->V : s->W;
     const double** Bd = which == 0 ? s->Vd : s->Wd;
     KrylovState& K = which == 0 ? s->KV : s->KW;
     const int ldh = s->mcap + 1;
-    std::vector<std::vector<double>*> Hs(s->S);
-    std::vector<double> tmpH;
-    for (int q = 0; q < s->S; ++q) {
+    const bool dc = (s->path & TSR_PATH_DCGS2) != 0;
+    const bool defer = (s->path & TSR_PATH_DEFERRED) != 0;
+    const double at = s->opt.AbsTol, rt = s->opt.RelTol;
+    std::vector<int> cap(S, 0), next(S, 0);
+    for (int q = 0; q < S; ++q) {
         ScState& c = s->sc[q];
-        Hs[q] = which == 0 ? &c.H : &c.HD;
         if (which == 0) { c.m = 0; c.bd = false; c.conv = false; c.phi_m = -1; } else { c.mD = 0; c.bdD = false; }
+        if (!s->mask_h[q]) continue;
+        cap[q] = std::min(which == 0 ? c.m_cap : s->opt.MaxKrylov, s->mcap);
+        const int pred = which == 0 ? c.pred_m : c.pred_mD;
+        next[q] = defer ? std::min(std::max(pred - 2, 1), cap[q]) : 1;
+        if (cap[q] < 1) s->mask_h[q] = 0;
     }
-    std::vector<std::vector<double>> Hv(s->S);
-    for (int j = 0; j < s->mcap; ++j) {
-        int any = 0;
-        for (int q = 0; q < s->S; ++q) {
-            const ScState& c = s->sc[q];
-            const int mcapq = which == 0 ? c.m_cap : s->opt.MaxKrylov;
-            if (s->mask_h[q] && j >= mcapq) s->mask_h[q] = 0;
-            any |= s->mask_h[q];
+    int j = 0;   // iterations enqueued so far: raw columns 1..j exist
+    while (true) {
+        int jt = INT_MAX;
+        for (int q = 0; q < S; ++q)
+            if (s->mask_h[q]) jt = std::min(jt, next[q]);
+        if (jt == INT_MAX) break;
+        if (upload_mask(s) || clear_bad(s)) return 2;
+        const int j0 = j;
+        for (; j < jt; ++j) {
+            if (G_call(s, B[j], K.veps, s->g0, K.ieps, B[j + 1], s->mask)) return 2;
+            if (dc) {
+                HIPCHK(launch_dcgs2(j + 1, Bd, B[j + 1], s->u, K, s->M, s->nblk, s->part, s->ldp, s->blk_first, S,
+                                    s->scen_rows, s->mask, at, rt, s->st));
+            } else {
+                HIPCHK(launch_pass(PASS_DOTS, Bd, j + 1, B[j + 1], s->u, K.coef, K.ld, s->M, s->nblk, s->part, s->ldp,
+                                   s->mask, at, rt, s->st));
+                HIPCHK(launch_reduce(RED_H1, s->part, s->ldp, j + 1, s->blk_first, S, s->scen_rows, K, j, s->mask, s->st));
+                HIPCHK(launch_pass(PASS_AXPY_DOTS, Bd, j + 1, B[j + 1], s->u, K.coef, K.ld, s->M, s->nblk, s->part,
+                                   s->ldp, s->mask, at, rt, s->st));
+                HIPCHK(launch_reduce(RED_H2, s->part, s->ldp, j + 1, s->blk_first, S, s->scen_rows, K, j, s->mask, s->st));
+                HIPCHK(launch_pass(PASS_AXPY_NORM, Bd, j + 1, B[j + 1], s->u, K.coef, K.ld, s->M, s->nblk, s->part,
+                                   s->ldp, s->mask, at, rt, s->st));
+            }
+            if (!dc)
+                HIPCHK(launch_reduce(RED_NORM, s->part, s->ldp, 2, s->blk_first, S, s->scen_rows, K, j, s->mask, s->st));
         }
-        if (!any) break;
-        if (upload_mask(s)) return 2;
-        if (clear_bad(s)) return 2;
-        if (G_call(s, B[j], K.veps, s->g0, K.ieps, B[j + 1], s->mask)) return 2;
-        HIPCHK(launch_pass(PASS_DOTS, Bd, j + 1, B[j + 1], s->u, K.coef, K.ld, s->M, s->nblk, s->part, s->ldp, s->mask,
-                           s->opt.AbsTol, s->opt.RelTol, s->st));
-        HIPCHK(launch_reduce(RED_H1, s->part, s->ldp, j + 1, s->blk_first, s->S, s->scen_rows, K, j, s->mask, s->st));
-        HIPCHK(launch_pass(PASS_AXPY_DOTS, Bd, j + 1, B[j + 1], s->u, K.coef, K.ld, s->M, s->nblk, s->part, s->ldp,
-                           s->mask, s->opt.AbsTol, s->opt.RelTol, s->st));
-        HIPCHK(launch_reduce(RED_H2, s->part, s->ldp, j + 1, s->blk_first, s->S, s->scen_rows, K, j, s->mask, s->st));
-        HIPCHK(launch_pass(PASS_AXPY_NORM, Bd, j + 1, B[j + 1], s->u, K.coef, K.ld, s->M, s->nblk, s->part, s->ldp,
-                           s->mask, s->opt.AbsTol, s->opt.RelTol, s->st));
-        HIPCHK(launch_reduce(RED_NORM, s->part, s->ldp, 2, s->blk_first, s->S, s->scen_rows, K, j, s->mask, s->st));
-        // gather H column j, the per-scenario results and the RHS flags of this iteration
-        {
-            double* tmp = s->pin_h;
-            HIPCHK(hipMemcpy2DAsync(tmp, sizeof(double) * (j + 2), K.H + (size_t)j * K.ldh,
-                                    sizeof(double) * K.hstride, sizeof(double) * (j + 2), s->S,
-                                    hipMemcpyDeviceToHost, s->st));
-            HIPCHK(hipMemcpyAsync(s->pin_res, K.res, sizeof(double) * 8 * s->S, hipMemcpyDeviceToHost, s->st));
-            HIPCHK(hipMemcpyAsync(s->pin_bad, s->bad, sizeof(int) * s->S, hipMemcpyDeviceToHost, s->st));
-            HIPCHK(timed_sync(s));
-            std::memcpy(s->res_h.data(), s->pin_res, sizeof(double) * 8 * s->S);
-            std::memcpy(s->bad_h.data(), s->pin_bad, sizeof(int) * s->S);
-            for (int q = 0; q < s->S; ++q)
-                if (s->mask_h[q])
-                    for (int i = 0; i <= j + 1; ++i) (*Hs[q])[(size_t)j * ldh + i] = tmp[(size_t)q * (j + 2) + i];
-        }
-        for (int q = 0; q < s->S; ++q) {
+        // H columns c0..jt-1 changed in this window (DCGS2 also corrects column j0-1)
+        const int c0 = dc ? std::max(j0 - 1, 0) : j0;
+        if (read_window(s, K, c0, jt)) return 2;
+        for (int q = 0; q < S; ++q) {
             if (!s->mask_h[q]) continue;
             ScState& c = s->sc[q];
-            c.fe++;
+            c.fe += jt - j0;
             if (s->bad_h[q]) { c.failed = true; s->mask_h[q] = 0; continue; }
-            const double hn = s->res_h[(size_t)q * 8 + 0];
-            const double wn = s->res_h[(size_t)q * 8 + 2] / hn;
-            const double wnorm0 = std::sqrt(s->res_h[(size_t)q * 8 + 1]);
-            const int m = j + 1;
-            if (which == 0) {
-                c.m = m; c.hn = hn; c.wnext = wn;
-                c.cs[j + 1] = 1.0 / hn;
-                if (!(hn > 1e-12 * wnorm0) || hn == 0.0) { c.bd = true; c.conv = true; s->mask_h[q] = 0; continue; }
-                if (m >= s->opt.MinKrylov && krylov_est_main(c, m, c.tau, ldh, s->mcap) <= kKrylovTol) {
-                    c.conv = true;
-                    s->mask_h[q] = 0;
-                }
-            } else {
-                c.mD = m; c.hnD = hn; c.wnextD = wn;
-                c.csW[j + 1] = 1.0 / hn;
-                if (!(hn > 1e-12 * wnorm0) || hn == 0.0) { c.bdD = true; s->mask_h[q] = 0; continue; }
-                if (m >= s->opt.MinKrylov) {
-                    std::vector<double> ph(3 * (size_t)m);
-                    int prc;
-                    {
-                        Tic tic(dense_acc());
-                        prc = phik_e1(ph.data(), c.HD.data(), ldh, m, c.tau, 3);
+            std::vector<double>& Hq = which == 0 ? c.H : c.HD;
+            std::vector<double>& csq = which == 0 ? c.cs : c.csW;
+            const double* hw = s->pin_h + (size_t)q * (jt - c0) * K.ldh;
+            for (int col = c0; col < jt; ++col)
+                for (int i = 0; i <= col + 1; ++i) Hq[(size_t)col * ldh + i] = hw[(size_t)(col - c0) * K.ldh + i];
+            std::memcpy(csq.data(), s->pin_cs + (size_t)q * (jt + 1), sizeof(double) * (jt + 1));
+            const double* nw = s->pin_nw + (size_t)q * jt;
+            const double* wr = s->pin_wr + (size_t)q * jt;
+            bool stop = false;
+            for (int m = j0 + 1; m <= jt && !stop; ++m) {
+                const double hn = Hq[(size_t)(m - 1) * ldh + m];
+                const double wnorm0 = std::sqrt(nw[m - 1]);
+                const double wn = wr[m - 1] / hn;
+                const bool bd = !(hn > 1e-12 * wnorm0) || hn == 0.0;
+                if (which == 0) {
+                    c.m = m; c.hn = hn; c.wnext = wn;
+                    if (bd) { c.bd = true; c.conv = true; stop = true; break; }
+                    if (m >= s->opt.MinKrylov && krylov_est_main(c, m, c.tau, ldh, s->mcap) <= kKrylovTol) {
+                        c.conv = true;
+                        stop = true;
                     }
-                    if (!prc) {
-                        const double est = 2.0 * c.tau * c.betaD * (c.tau * hn * std::fabs(ph[2 * m + m - 1])) * wn;
-                        if (est <= kKrylovTol) s->mask_h[q] = 0;
+                } else {
+                    c.mD = m; c.hnD = hn; c.wnextD = wn;
+                    if (bd) { c.bdD = true; stop = true; break; }
+                    if (m >= s->opt.MinKrylov) {
+                        std::vector<double> ph(3 * (size_t)m);
+                        int prc;
+                        {
+                            Tic tic(dense_acc());
+                            prc = phik_e1(ph.data(), c.HD.data(), ldh, m, c.tau, 3);
+                        }
+                        if (!prc) {
+                            const double est = 2.0 * c.tau * c.betaD * (c.tau * hn * std::fabs(ph[2 * m + m - 1])) * wn;
+                            if (est <= kKrylovTol) stop = true;
+                        }
                     }
                 }
             }
+            if (stop || jt >= cap[q]) s->mask_h[q] = 0;
+            else next[q] = jt + 1;
         }
     }
     return 0;
 }
 
-int features_and_proposal(tsr_solver* s);
+int features_and_proposal(tsr_solver* s, const std::vector<int>& act);
 
 }  // namespace
 
@@ -342,6 +396,7 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
     const int S = s->S, ldh = s->mcap + 1;
     std::vector<int> act(S, 0);
     int nact = 0;
+    if (rec) std::memset(rec, 0, sizeof(tsr_step_record) * S);   // finished scenarios: all-zero records
     for (int q = 0; q < S; ++q) {
         ScState& c = s->sc[q];
         c.failed = false; c.accepted = false; c.rejects = 0; c.fe = 0; c.m = 0; c.mD = 0; c.err = 0.0; c.ctrl = 0;
@@ -349,20 +404,6 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
         if (c.stats.NumSteps >= o.MaxNumSteps) return -1;
     }
     if (!nact) return 0;
-    if (s->ctrl_mode == 1 && s->mlp && mlp_ready(s->mlp)) {
-        if (features_and_proposal(s)) return -1;
-    }
-    for (int q = 0; q < S; ++q) {
-        if (!act[q]) continue;
-        ScState& c = s->sc[q];
-        if (c.ctrl == 0) { c.m_cap = o.MaxKrylov; }
-        c.proposal = std::fmin(std::fmax(c.proposal, o.MinStep), o.MaxStep);
-        c.tstop = tend;
-        c.tau = c.proposal;
-        c.tau_prop = c.proposal;
-        c.landing = false;
-        if (c.t + c.tau >= c.tstop - 1e-13 * std::fabs(c.tstop)) { c.tau = c.tstop - c.t; c.landing = true; }
-    }
     s->mask_h = act;
     if (upload_mask(s) || clear_bad(s)) return -1;
     if (G_call(s, nullptr, nullptr, nullptr, nullptr, s->g0, s->mask)) return -1;
@@ -370,9 +411,27 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
         return -1;
     if (launch_reduce(RED_START, s->part, s->ldp, 3, s->blk_first, S, s->scen_rows, s->KV, 0, s->mask, s->st) != hipSuccess)
         return -1;
+    HIPCHK(launch_macro_stats(batch_of(s->P).scen, S, s->u, s->mstats, s->st));
+    HIPCHK(hipMemcpyAsync(s->pin_ms, s->mstats, sizeof(double) * 6 * S, hipMemcpyDeviceToHost, s->st));
     std::vector<double> eps_h(S);
     HIPCHK(hipMemcpyAsync(eps_h.data(), s->KV.eps, sizeof(double) * S, hipMemcpyDeviceToHost, s->st));
     if (read_iteration(s, s->KV, 0, nullptr)) return -1;
+    for (int q = 0; q < S; ++q) std::memcpy(s->sc[q].mstat, s->pin_ms + (size_t)q * 6, sizeof(double) * 6);
+    // proposal: the classical law's (kept in c.proposal), replaced by the learned
+    // controller's when it is active and its output is usable
+    if (features_and_proposal(s, act)) return -1;
+    for (int q = 0; q < S; ++q) {
+        if (!act[q]) continue;
+        ScState& c = s->sc[q];
+        if (c.ctrl != 1) { c.m_cap = o.MaxKrylov; }
+        c.proposal = std::fmin(std::fmax(c.proposal, o.MinStep), o.MaxStep);
+        c.tstop = tend;
+        c.tau = c.proposal;
+        c.tau_prop = c.proposal;
+        c.landing = false;
+        c.first_done = false;
+        if (c.t + c.tau >= c.tstop - 1e-13 * std::fabs(c.tstop)) { c.tau = c.tstop - c.t; c.landing = true; }
+    }
     for (int q = 0; q < S; ++q) {
         if (!act[q]) continue;
         ScState& c = s->sc[q];
@@ -518,6 +577,11 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
             ScState& c = s->sc[q];
             double err = dq[q] ? std::sqrt(s->res_h[(size_t)q * 8 + 0] / (double)s->rows_h[q]) : 0.0;
             c.err = err;
+            if (!c.first_done) {
+                c.first_done = true;
+                c.tau_first = c.tau;
+                c.err_first = okq[q] ? err : INFINITY;
+            }
             if (okq[q] && std::isfinite(err) && err <= 1.0) { acc[q] = 1; ++nacc; continue; }
             c.rejects++;
             const double fac = (okq[q] && std::isfinite(err)) ? std::fmax(o.MinStepDecFac, o.SafetyFac * std::pow(err, -1.0 / 3.0))
@@ -555,21 +619,30 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
         c.stats.CurrentKryDim = c.m;
         c.stats.NumSteps += 1;
         c.stats.FuncEvals += c.fe;
+        double hnorm = 0.0;
+        for (int i = 0; i < c.m; ++i) {
+            double rs = 0.0;
+            for (int k = 0; k < c.m; ++k) rs += std::fabs(c.H[i + (size_t)k * ldh]);
+            hnorm = std::max(hnorm, rs);
+        }
         for (int k = 3; k > 0; --k) { c.hist_tau[k] = c.hist_tau[k - 1]; c.hist_m[k] = c.hist_m[k - 1]; }
         c.hist_tau[0] = taken;
         c.hist_m[0] = c.m;
+        c.pred_m = c.m;
+        c.pred_mD = c.mD;
+        c.n_hist = std::min(c.n_hist + 1, 4);
+        c.last_hnorm = hnorm;
+        c.last_resid = c.hn;
         if (rec) {
             tsr_step_record& r = rec[q];
             std::memset(&r, 0, sizeof(r));
             r.t = c.t; r.tau = taken; r.tau_proposed = c.tau_prop; r.err = c.err; r.beta = c.beta;
-            double hnorm = 0.0;
-            for (int i = 0; i < c.m; ++i) {
-                double rs = 0.0;
-                for (int k = 0; k < c.m; ++k) rs += std::fabs(c.H[i + (size_t)k * ldh]);
-                hnorm = std::max(hnorm, rs);
-            }
             r.hnorm = hnorm; r.resid = c.hn; r.m = c.m; r.mD = c.mD; r.rejects = c.rejects; r.fevals = c.fe;
             r.controller = c.ctrl; r.m_cap = c.m_cap;
+            r.head_mean = c.mstat[0]; r.head_min = c.mstat[1]; r.head_max = c.mstat[2];
+            r.sat_mean = c.mstat[3]; r.sat_min = c.mstat[4]; r.sat_max = c.mstat[5];
+            r.tau_first = c.tau_first; r.err_first = c.err_first;
+            std::memcpy(r.feat, c.feat, sizeof(r.feat));
         }
     }
     s->mask_h.assign(S, 1);
@@ -578,37 +651,57 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
 
 namespace {
 
-// Learned-controller features (16 per scenario) and proposal; classical values are kept
-// when the network's output is not finite.
-int features_and_proposal(tsr_solver* s)
+// Controller inputs (16 per scenario, order of TSR_FEAT_COUNT in tsr.h) and, with the
+// learned controller active, its proposal: step = kLearnedSafety * 10^y0, clamped to
+// [1/kLearnedTrust, kLearnedTrust] x the classical proposal; Krylov cap from y1.  A
+// scenario whose network output is not finite keeps the classical proposal (ctrl = 2).
+constexpr double kLearnedSafety = 0.8;
+constexpr double kLearnedTrust = 8.0;
+
+void build_features(const ScState& c, float* f)
+{
+    const double lp = std::log10(std::max(c.proposal, 1e-300));
+    f[0] = (float)lp;
+    double last = lp;
+    for (int k = 0; k < 3; ++k) {
+        if (k < c.n_hist) last = std::log10(std::max(c.hist_tau[k], 1e-300));
+        f[1 + k] = (float)last;
+    }
+    double lm = 0.0;
+    for (int k = 0; k < 4; ++k) {
+        if (k < c.n_hist) lm = c.hist_m[k];
+        f[4 + k] = (float)lm;
+    }
+    f[8] = c.n_hist ? (float)std::log10(std::max(c.last_hnorm, 1e-300)) : 0.0f;
+    f[9] = c.n_hist ? (float)std::log10(std::max(c.last_resid, 1e-300)) : 0.0f;
+    for (int k = 0; k < 6; ++k) f[10 + k] = (float)c.mstat[k];
+}
+
+int features_and_proposal(tsr_solver* s, const std::vector<int>& act)
 {
     const int S = s->S;
-    const tsr_batch_view* vv = nullptr;
-    (void)vv;
-    std::vector<float> feat((size_t)S * 16), out((size_t)S * 2);
-    const int fail_code = -1;
     for (int q = 0; q < S; ++q) {
-        ScState& c = s->sc[q];
-        float* f = &feat[(size_t)q * 16];
-        for (int k = 0; k < 4; ++k) f[k] = (float)std::log(std::max(c.hist_tau[k] > 0 ? c.hist_tau[k] : c.proposal, 1e-12));
-        for (int k = 0; k < 4; ++k) f[4 + k] = (float)(c.hist_m[k] > 0 ? c.hist_m[k] : 3.0);
-        f[8] = 0.0f; f[9] = 0.0f;
-        for (int k = 10; k < 16; ++k) f[k] = 0.0f;
+        s->sc[q].ctrl = 0;
+        if (act[q]) build_features(s->sc[q], s->sc[q].feat);
     }
-    if (!s->macro_h.empty()) {
-        // macro head / saturation statistics are filled by the caller through macro_h
-    }
-    if (mlp_infer(s->mlp, feat.data(), S, out.data(), s->st)) return -1;
+    if (!(s->ctrl_mode == 1 && s->mlp && mlp_ready(s->mlp))) return 0;
+    std::vector<float> feat((size_t)S * 16, 0.0f), out((size_t)S * 2, 0.0f);
+    for (int q = 0; q < S; ++q) std::memcpy(&feat[(size_t)q * 16], s->sc[q].feat, sizeof(float) * 16);
+    if (mlp_infer(s->mlp, feat.data(), S, out.data(), s->st)) { set_error("controller inference failed"); return -1; }
+    const int hi_cap = std::min(kLearnedCap, s->mcap);
     for (int q = 0; q < S; ++q) {
+        if (!act[q]) continue;
         ScState& c = s->sc[q];
-        const double lt = out[(size_t)q * 2], mk = out[(size_t)q * 2 + 1];
-        if (std::isfinite(lt) && std::isfinite(mk)) {
-            c.proposal = std::exp(lt);
-            c.m_cap = std::min(std::max((int)std::lround(mk), 5), std::min(60, s->mcap));
+        const double y0 = out[(size_t)q * 2], y1 = out[(size_t)q * 2 + 1];
+        if (std::isfinite(y0) && std::isfinite(y1) && std::fabs(y0) < 300.0) {
+            const double tl = kLearnedSafety * std::pow(10.0, y0);
+            c.proposal = std::fmin(std::fmax(tl, c.proposal / kLearnedTrust), c.proposal * kLearnedTrust);
+            const int mh = std::min(std::max((int)std::ceil(std::fmin(std::fmax(y1, 0.0), 1000.0)), 5), hi_cap);
+            c.m_cap = std::min(std::max(mh + std::max(4, mh / 4), std::max(5, s->opt.MinKrylov)), hi_cap);
+            c.pred_m = mh;
             c.ctrl = 1;
         } else {
             c.ctrl = 2;
-            c.m_cap = s->opt.MaxKrylov;
         }
     }
     return 0;
@@ -671,13 +764,15 @@ tsr_solver* create(const tsr_batch_view* view, const exprem_options_struct* opt)
     hipMemcpy(s->scen_rows, s->rows_h.data(), sizeof(long long) * S, hipMemcpyHostToDevice);
     s->M.scen = d_bs; s->M.r0 = d_b0; s->M.r1 = d_b1;
     s->ldp = s->nblk;   // partials are column-major: part[k * nblk + b], k < mcap + 2
-    if (dalloc(&s->part, (size_t)s->nblk * (s->mcap + 2))) return nullptr;
+    if (dalloc(&s->part, (size_t)s->nblk * (2 * s->mcap + 4))) return nullptr;   // DCGS2 needs 2 (j+1) + 1 columns
     auto kstate = [&](KrylovState& K, double* shared_eps, double* shared_ieps) {
         K.ld = s->mcap + 2;
         K.ldh = s->mcap + 1;
         K.hstride = (s->mcap + 1) * s->mcap;
+        K.ld2 = 2 * K.ld + 2;
         int r = dalloc(&K.cs, (size_t)S * K.ld) | dalloc(&K.coef, (size_t)S * K.ld) | dalloc(&K.H, (size_t)S * K.hstride) |
-                dalloc(&K.veps, S) | dalloc(&K.res, (size_t)S * 8);
+                dalloc(&K.veps, S) | dalloc(&K.res, (size_t)S * 8) | dalloc(&K.coefr, (size_t)S * K.ld) |
+                dalloc(&K.dots, (size_t)S * K.ld2) | dalloc(&K.nwj, (size_t)S * K.ld) | dalloc(&K.wrj, (size_t)S * K.ld);
         if (shared_eps) { K.eps = shared_eps; K.ieps = shared_ieps; }
         else r |= dalloc(&K.eps, S) | dalloc(&K.ieps, S);
         return r;
@@ -701,9 +796,14 @@ tsr_solver* create(const tsr_batch_view* view, const exprem_options_struct* opt)
         c.cs.assign(s->mcap + 2, 0.0);
         c.csW.assign(s->mcap + 2, 0.0);
     }
-    if (hipHostMalloc((void**)&s->pin_h, sizeof(double) * (size_t)S * (s->mcap + 2)) != hipSuccess ||
+    if (hipHostMalloc((void**)&s->pin_h, sizeof(double) * (size_t)S * s->mcap * (s->mcap + 1)) != hipSuccess ||
+        hipHostMalloc((void**)&s->pin_cs, sizeof(double) * (size_t)S * (s->mcap + 2)) != hipSuccess ||
+        hipHostMalloc((void**)&s->pin_nw, sizeof(double) * (size_t)S * (s->mcap + 2)) != hipSuccess ||
+        hipHostMalloc((void**)&s->pin_wr, sizeof(double) * (size_t)S * (s->mcap + 2)) != hipSuccess ||
         hipHostMalloc((void**)&s->pin_res, sizeof(double) * (size_t)S * 8) != hipSuccess ||
-        hipHostMalloc((void**)&s->pin_bad, sizeof(int) * (size_t)S) != hipSuccess) {
+        hipHostMalloc((void**)&s->pin_bad, sizeof(int) * (size_t)S) != hipSuccess ||
+        hipHostMalloc((void**)&s->pin_ms, sizeof(double) * (size_t)S * 6) != hipSuccess ||
+        dalloc(&s->mstats, (size_t)S * 6)) {
         set_error("hipHostMalloc failed");
         return nullptr;
     }
@@ -737,12 +837,21 @@ void tsr_solver_free(tsr_solver* s)
     hipFree(s->Vd); hipFree(s->Wd);
     hipFree((void*)s->M.scen); hipFree((void*)s->M.r0); hipFree((void*)s->M.r1);
     hipFree(s->blk_first); hipFree(s->scen_rows); hipFree(s->part);
-    for (KrylovState* K : {&s->KV, &s->KW}) { hipFree(K->cs); hipFree(K->coef); hipFree(K->H); hipFree(K->veps); hipFree(K->res); }
+    for (KrylovState* K : {&s->KV, &s->KW}) {
+        hipFree(K->cs); hipFree(K->coef); hipFree(K->H); hipFree(K->veps); hipFree(K->res);
+        hipFree(K->coefr); hipFree(K->dots); hipFree(K->nwj); hipFree(K->wrj);
+    }
     hipFree(s->KV.eps); hipFree(s->KV.ieps);
     hipFree(s->mask); hipFree(s->nk);
     if (s->pin_h) hipHostFree(s->pin_h);
     if (s->pin_res) hipHostFree(s->pin_res);
     if (s->pin_bad) hipHostFree(s->pin_bad);
+    if (s->pin_ms) hipHostFree(s->pin_ms);
+    if (s->pin_cs) hipHostFree(s->pin_cs);
+    if (s->pin_nw) hipHostFree(s->pin_nw);
+    if (s->pin_wr) hipHostFree(s->pin_wr);
+    if (s->linrec) hipFree(s->linrec);
+    if (s->mstats) hipFree(s->mstats);
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
     if (s->mlp) mlp_free(s->mlp);
@@ -758,6 +867,7 @@ void tsr_solver_reset(tsr_solver* s)
         c.proposal = s->opt.InitStep;
         exprem_stats_init(&c.stats, &s->opt, nullptr);
         for (int k = 0; k < 4; ++k) { c.hist_tau[k] = 0.0; c.hist_m[k] = 0.0; }
+        c.n_hist = 0; c.last_hnorm = 0.0; c.last_resid = 0.0; c.pred_m = 0; c.pred_mD = 0;
     }
 }
 
@@ -890,6 +1000,45 @@ int tsr_solver_run(tsr_solver* s, double tend, int max_steps, tsr_run_stats* out
     }
     g_timed = nullptr;
     return rc;
+}
+
+int tsr_solver_jv(tsr_solver* s, const double* v_host, double eps, int lin, double* jv_host)
+{
+    const int fail_code = 2;
+    if (lin && !s->linrec && tsr_solver_set_path(s, s->path | TSR_PATH_JVLIN)) return 2;
+    const size_t N = (size_t)s->N;
+    std::vector<double> e(2 * (size_t)s->S);
+    for (int q = 0; q < s->S; ++q) { e[q] = eps; e[s->S + q] = 1.0 / eps; }
+    double* de = nullptr;
+    if (hipMalloc((void**)&de, sizeof(double) * e.size()) != hipSuccess) { set_error("hipMalloc failed"); return 2; }
+    HIPCHK(hipMemcpyAsync(de, e.data(), sizeof(double) * e.size(), hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMemcpyAsync(s->scratch[0], v_host, sizeof(double) * N, hipMemcpyHostToDevice, s->st));
+    GArgs a;
+    a.u = s->u; a.v = nullptr; a.veps = nullptr; a.g0 = nullptr; a.ieps = nullptr; a.active = nullptr; a.out = s->g0;
+    a.rec = lin ? s->linrec : nullptr;
+    HIPCHK(launch_gfunc(s->P, a, s->st));
+    a.v = s->scratch[0]; a.veps = de; a.g0 = s->g0; a.ieps = de + s->S; a.out = s->scratch[1];
+    a.rec = nullptr;
+    a.lin = lin ? s->linrec : nullptr;
+    HIPCHK(launch_gfunc(s->P, a, s->st));
+    HIPCHK(hipMemcpyAsync(jv_host, s->scratch[1], sizeof(double) * N, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    hipFree(de);
+    return 0;
+}
+
+int tsr_solver_set_path(tsr_solver* s, int path)
+{
+    if (path & ~TSR_PATH_BATCHED) return 1;
+    if ((path & TSR_PATH_JVLIN) && !s->linrec) {
+        if (hipMalloc((void**)&s->linrec, sizeof(double) * kLinComp * (size_t)s->N) != hipSuccess) {
+            s->linrec = nullptr;
+            set_error("hipMalloc failed for the linearisation record");
+            return 2;
+        }
+    }
+    s->path = path;
+    return 0;
 }
 
 int tsr_solver_set_controller(tsr_solver* s, int mode)
