@@ -130,7 +130,7 @@ typedef struct tsr_step_record {
     int    mD;           /* Krylov dimension of the error estimate */
     int    rejects;      /* rejected attempts before acceptance */
     int    fevals;       /* RHS evaluations in this step */
-    int    controller;   /* 0 classical, 1 learned proposal accepted, 2 learned proposal fell back */
+    int    controller;   /* 0 classical, 1 learned proposal, 2 learned output unusable (classical), 3 exploration */
     int    m_cap;        /* Krylov cap used */
 } tsr_step_record;
 
@@ -167,6 +167,10 @@ void tsr_solver_micro_shape(const tsr_solver* s, int* bs, int* wpe);
 /* Average device time (ms) of each of G's three kernels (macro elements, micro cells,
  * macro nodes) over `reps` evaluations of G(u_dev), HIP events between the launches. */
 int tsr_solver_time_gfunc_parts(tsr_solver* s, double* g_dev, const double* u_dev, int reps, double* ms3);
+/* As tsr_solver_time_gfunc_parts for one k_micro mode: 0 G, 1 G recording the
+ * linearisation point, 2 an FD product with the linearised constitutive law (after a
+ * mode-1 evaluation at u_dev).  Returns 1 on an unknown mode. */
+int tsr_solver_time_gfunc_mode(tsr_solver* s, double* g_dev, const double* u_dev, int mode, int reps, double* ms3);
 /* Average device time (ms) of a plain copy of N doubles (reads 8N, writes 8N bytes):
  * the calibration kernel for HBM traffic counters. */
 int tsr_solver_time_copy(tsr_solver* s, int reps, double* ms);
@@ -177,9 +181,13 @@ int tsr_solver_step(tsr_solver* s, double tend, tsr_step_record* rec);
 int tsr_solver_run(tsr_solver* s, double tend, int max_steps, tsr_run_stats* out, tsr_step_record* records,
                    int max_records);
 
-/* Step controller: 0 classical (EXPREM's law), 1 learned (MLP proposal, classical fallback).
- * tsr_solver_set_mlp loads FP32 master weights (layout of tsr_mlp_param_count). */
+/* Step controller: 0 classical (EXPREM's law), 1 learned (MLP proposal, classical fallback),
+ * 2 exploration (random walk keeping the Krylov dimension in [3, 24]; generates the
+ * learned controller's training data).  Returns 1 on an unknown mode.
+ * tsr_solver_set_mlp loads FP32 master weights (layout of tsr_mlp_param_count) and the
+ * feature standardisation; tsr_solver_set_seed seeds the exploration controller. */
 int tsr_solver_set_controller(tsr_solver* s, int mode);
+void tsr_solver_set_seed(tsr_solver* s, unsigned long long seed);
 int tsr_solver_set_mlp(tsr_solver* s, const float* params, const float* feat_mean, const float* feat_std);
 
 /* The learned controller's network, 16 -> 64 -> 64 -> 2 with GELU; BF16 weights (round

@@ -131,6 +131,7 @@ def declare(lib):
         "tsr_solver_gfunc_dev": (C.c_int, [P, C.c_void_p, C.c_void_p, C.c_int]),
         "tsr_solver_last_gfunc_ms": (C.c_double, [P]),
         "tsr_solver_time_gfunc_parts": (C.c_int, [P, C.c_void_p, C.c_void_p, C.c_int, _dp]),
+        "tsr_solver_time_gfunc_mode": (C.c_int, [P, C.c_void_p, C.c_void_p, C.c_int, C.c_int, _dp]),
         "tsr_solver_time_copy": (C.c_int, [P, C.c_int, _dp]),
         "tsr_solver_micro_shape": (None, [P, _ip, _ip]),
         "tsr_solver_step": (C.c_int, [P, C.c_double, C.POINTER(tsr_step_record)]),
@@ -138,6 +139,7 @@ def declare(lib):
                                      C.POINTER(tsr_step_record), C.c_int]),
         "tsr_solver_set_controller": (C.c_int, [P, C.c_int]),
         "tsr_solver_set_path": (C.c_int, [P, C.c_int]),
+        "tsr_solver_set_seed": (None, [P, C.c_ulonglong]),
         "tsr_solver_jv": (C.c_int, [P, _dp, C.c_double, C.c_int, _dp]),
         "tsr_solver_set_mlp": (C.c_int, [P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "tsr_mlp_param_count": (C.c_int, []),

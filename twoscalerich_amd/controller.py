@@ -4,22 +4,36 @@ with the HIP kernels of csrc/mlp.hip, and attachment to a solver.
 The reference's step law (src/set_exprem_options.cpp:20-26 options; the EXPREM driver
 include/main.h:29-32) is the classical one,
     tau_next = tau * min(MaxStepIncFac, max(MinStepDecFac, SafetyFac * err^(-1/3))),
-which only sees the previous step's error.  The learned controller replaces the
-proposal (step size and Krylov budget) at the start of a step; EXPREM's own error
-estimate still accepts or rejects the attempt, and the classical law is the fallback
-whenever the network's output is unusable.
+which only sees the previous step's error.  On these problems the error estimate is
+rarely what limits the step: the Krylov stopping rule is.  The classical law grows the
+step by MaxStepIncFac until the Arnoldi process reaches its cap, halves it, and grows
+it again, and the Krylov dimension needed rises steeply along that saw-tooth (on the
+plain workload from about 6 to 30 while the step grows by less than 2x), so most of the
+work goes into the largest steps.
 
-Inputs: the 16 features the solver writes into every tsr_step_record (order in
-include/tsr/tsr.h).  Targets, from the same record:
-    y0 = log10(tau_first * err_first^(-1/3))   the step at which the first attempt's
-                                               error would have been exactly 1
-                                               (local error ~ tau^3: exprb32 is of order 2)
-    y1 = m                                     the Krylov dimension the step needed
-Rejected first attempts (err_first > 1) are training samples like any other: their
-target is the smaller step they should have tried.  Features are standardised with
-the training set's mean and standard deviation; targets are standardised with a
-power-of-two scale, so folding the target scale back into the output layer is exact in
-BF16.
+The learned controller proposes, at the start of each step, the step size with the
+lowest expected cost per unit of simulated time, and the Krylov dimension it expects
+that step to need (the window of the deferred stopping-rule checks).  EXPREM's own error
+estimate still accepts or rejects each attempt, the Krylov rule still halves a step
+whose basis reaches the cap, and the classical law is the fallback whenever the
+network's output is unusable.
+
+Training data: step records (tsr_step_record: the 16 features at the step start, the
+first attempt's step and error, the Krylov dimensions) from runs driven by the
+exploration controller (controller mode 2), which random-walks the step size so that
+nearby steps sample the cost curve.  Labels, per step k, from the records of the same
+scenario within +-`window` steps (the state changes little over that span):
+    for every candidate step size tau_c among them, the Krylov dimensions m(tau_c) and
+    mD(tau_c) are the medians over records with |log10 tau - log10 tau_c| <= 0.05; the
+    cost of a step is (in units of one linearised RHS evaluation)
+        cost(m, mD) = c0 + m + mD + r (m^2 + 8 m + mD^2 + 5 mD)
+    (RHS evaluations, plus r per basis column read in the Gram-Schmidt passes); the
+    label is the tau_c with the largest tau_c / cost, among candidates whose first
+    attempt's error was at most `err_max`:
+        y0 = log10 tau_c,  y1 = m(tau_c).
+Features are standardised with the training set's mean and standard deviation;
+targets with a power-of-two scale, so folding the target scale back into the output
+layer is exact in BF16.
 """
 import ctypes as C
 import os
@@ -55,12 +69,69 @@ def records_to_arrays(recs):
     return X, cols
 
 
-def targets(cols):
-    """(targets [n,2] float32, usable mask) from record fields."""
-    tf, ef, m = cols["tau_first"], cols["err_first"], cols["m"]
-    ok = np.isfinite(ef) & (ef > 0) & (tf > 0) & (m > 0)
-    y0 = np.where(ok, np.log10(np.where(ok, tf, 1.0)) - np.log10(np.where(ok, ef, 1.0)) / 3.0, 0.0)
-    return np.stack([y0, m], 1).astype(np.float32), ok
+# cost model of one step on the batched path, in units of one linearised RHS evaluation
+COST_C0 = 5.5      # G(u) with the expansion record, G(U2), the combinations, host syncs
+COST_R = 0.064     # one basis-column read in a Gram-Schmidt pass / one RHS evaluation
+
+
+def step_cost(m, mD, c0=COST_C0, r=COST_R):
+    return c0 + m + mD + r * (m * m + 8.0 * m + mD * mD + 5.0 * mD)
+
+
+def label_sequence(cols, window=20, err_max=0.7, c0=COST_C0, r=COST_R):
+    """Labels for one scenario's consecutive records (dict of arrays, in step order).
+    Returns (targets [n,2] float32, usable mask)."""
+    tf, ef = cols["tau_first"], cols["err_first"]
+    m, mD = cols["m"], cols["mD"]
+    n = len(tf)
+    ok = np.isfinite(ef) & (tf > 0) & (m > 0)
+    lt = np.where(ok, np.log10(np.where(tf > 0, tf, 1.0)), 0.0)
+    Y = np.zeros((n, 2), np.float32)
+    good = np.zeros(n, bool)
+    for k in range(n):
+        if not ok[k]:
+            continue
+        lo, hi = max(0, k - window), min(n, k + window + 1)
+        idx = np.arange(lo, hi)
+        idx = idx[ok[idx]]
+        if len(idx) < 5:
+            continue
+        best, by = -np.inf, None
+        for c in idx:
+            if ef[c] > err_max:
+                continue
+            near = idx[np.abs(lt[idx] - lt[c]) <= 0.05]
+            mm = float(np.median(m[near]))
+            mdd = float(np.median(mD[near]))
+            thr = tf[c] / step_cost(mm, mdd, c0, r)
+            if thr > best:
+                best, by = thr, (lt[c], mm)
+        if by is not None:
+            Y[k] = by
+            good[k] = True
+    return Y, good
+
+
+def records_by_scenario(recs, S):
+    """Split a step-major record list (S records per solver step) into per-scenario
+    sequences, dropping the all-zero records of finished scenarios."""
+    seqs = []
+    for q in range(S):
+        rq = [r for r in recs[q::S] if r.tau > 0]
+        if rq:
+            seqs.append(rq)
+    return seqs
+
+
+def dataset(seqs, **kw):
+    """(X [n,16], Y [n,2]) over per-scenario record sequences."""
+    Xs, Ys = [], []
+    for rq in seqs:
+        X, cols = records_to_arrays(rq)
+        Y, good = label_sequence(cols, **kw)
+        Xs.append(X[good])
+        Ys.append(Y[good])
+    return np.concatenate(Xs), np.concatenate(Ys)
 
 
 def init_params(seed=0):
@@ -200,3 +271,141 @@ def collect(solver, tend, max_steps, chunk=500, progress=None):
         if st.steps == 0 or st.t >= tend:
             break
     return out
+
+
+# ------------------------------------------------------------------ training pipeline
+# (workload, seed, max solver steps): the plain and ragged workloads from t = 0, and
+# three batches of 64 scenarios with sampled soils (192 soil draws).
+PLAN = ((F.CFG_PLAIN, 0, 30000), (F.CFG_RAGGED, 0, 6000),
+        (F.CFG_SCENARIOS, 11, 1200), (F.CFG_SCENARIOS, 12, 1200), (F.CFG_SCENARIOS, 13, 1200))
+
+
+def generate(plan=PLAN, path=F.PATH_BATCHED, log=print):
+    """Run the exploration controller over the plan; returns arrays
+    {X [n,16], fields [n,len(REC_FIELDS)], seq [n]} (seq = scenario sequence id)."""
+    from . import Problem, Solver
+    Xs, Fs, Ss = [], [], []
+    sid = 0
+    for cfg, seed, nmax in plan:
+        p = Problem.config(cfg, seed=seed)
+        s = Solver(p, p.options())
+        s.set_path(path)
+        if s.set_controller(2):
+            raise RuntimeError("exploration controller unavailable")
+        lib().tsr_solver_set_seed(s.h, 7919 * (seed + 1) + cfg)
+        s.reset()
+        s.set_state(p.initial(), 0.0)
+        S = p.view.contents.n_scen
+
+        def prog(st, n):
+            log(f"[controller] workload {cfg} seed {seed}: {n} steps, t = {st.t:.4e}")
+
+        recs = collect(s, p.tend(), nmax, progress=prog)
+        for rq in records_by_scenario(recs, S):
+            X, cols = records_to_arrays(rq)
+            Xs.append(X)
+            Fs.append(np.stack([cols[k] for k in REC_FIELDS], 1))
+            Ss.append(np.full(len(rq), sid))
+            sid += 1
+        s.close()
+        p.close()
+    return {"X": np.concatenate(Xs), "fields": np.concatenate(Fs), "seq": np.concatenate(Ss)}
+
+
+def dataset_from_arrays(d, **kw):
+    """Labelled (X, Y) from generate()'s arrays."""
+    X, Fd, seq = d["X"], d["fields"], d["seq"]
+    Xs, Ys = [], []
+    for q in np.unique(seq):
+        sel = seq == q
+        cols = {k: Fd[sel, i] for i, k in enumerate(REC_FIELDS)}
+        Y, good = label_sequence(cols, **kw)
+        Xs.append(X[sel][good])
+        Ys.append(Y[good])
+    return np.concatenate(Xs), np.concatenate(Ys)
+
+
+def summarize(workload, controller, path, tend=0.0, max_steps=10 ** 7, log=print):
+    """Run one workload to t_end with a controller ("classical", "learned", "explore") and
+    a solver path; return a dict of step statistics (the controller's view of the run)."""
+    import time
+    from . import Problem, Solver
+    p = Problem.config(workload)
+    s = Solver(p, p.options())
+    s.set_path(path)
+    if controller == "learned":
+        attach(s)
+    elif controller == "explore":
+        s.set_controller(2)
+        lib().tsr_solver_set_seed(s.h, 5)
+    tend = tend if tend > 0 else p.tend()
+    s.reset()
+    s.set_state(p.initial(), 0.0)
+    s.run(min(tend, 1e-9), 2)                      # kernel tuning outside the timed run
+    s.reset()
+    s.set_state(p.initial(), 0.0)
+    lib().tsr_device_sync()
+    t0 = time.perf_counter()
+    recs = collect(s, tend, max_steps, progress=lambda st, n: log(f"[controller] {n} steps, t = {st.t:.4e}"))
+    lib().tsr_device_sync()
+    wall = time.perf_counter() - t0
+    u, t = s.get_state()
+    s.close()
+    X, c = records_to_arrays(recs)
+    m = c["m"].astype(int)
+    prev = c["tau"][:-1]
+    ratio = c["tau_proposed"][1:] / np.where(prev > 0, prev, 1.0)
+    out = {"workload": workload, "controller": controller, "path": path, "t": t, "wall_s": wall,
+           "steps": len(recs), "rejects": int(c["rejects"].sum()), "fevals": int(c["fevals"].sum()),
+           "fevals_per_step": float(c["fevals"].mean()), "mean_m": float(m.mean()), "mean_mD": float(c["mD"].mean()),
+           "steps_at_krylov_cap": int(np.sum(m >= c["m_cap"])),
+           "krylov_halvings": int(np.sum((c["tau_first"] < c["tau_proposed"] * 0.999))),
+           "proposals_capped_1.1": float(np.mean(np.abs(ratio - 1.1) < 1e-9)) if len(ratio) else 0.0,
+           "err_quantiles_10_50_90": [float(q) for q in np.quantile(c["err"], [0.1, 0.5, 0.9])],
+           "median_tau": float(np.median(c["tau"])), "m_histogram": np.bincount(m).tolist()}
+    p.close()
+    return out, u
+
+
+def main(argv=None):
+    import argparse
+    import json
+    import time
+    ap = argparse.ArgumentParser(description="learned step controller: data generation and training")
+    ap.add_argument("--data", default="", help="records file (.npz) to write (generate) or read (train)")
+    ap.add_argument("--generate", action="store_true")
+    ap.add_argument("--train", action="store_true")
+    ap.add_argument("--summarize", action="store_true", help="run a workload and print step statistics")
+    ap.add_argument("--workload", type=int, default=F.CFG_PLAIN)
+    ap.add_argument("--controller", default="classical", choices=["classical", "learned", "explore"])
+    ap.add_argument("--path", type=int, default=F.PATH_PLAIN)
+    ap.add_argument("--tend", type=float, default=0.0)
+    ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--max-samples", type=int, default=60000)
+    a = ap.parse_args(argv)
+    if a.summarize:
+        out, _ = summarize(a.workload, a.controller, a.path, a.tend)
+        print(json.dumps(out), flush=True)
+    if a.generate:
+        t0 = time.time()
+        d = generate()
+        np.savez(a.data, **d)
+        print(f"[controller] {len(d['X'])} records in {time.time() - t0:.1f} s -> {a.data}", flush=True)
+    if a.train:
+        with np.load(a.data, allow_pickle=False) as z:
+            d = {k: z[k] for k in z.files}
+        X, Y = dataset_from_arrays(d)
+        rng = np.random.default_rng(0)
+        if len(X) > a.max_samples:
+            keep = rng.choice(len(X), a.max_samples, replace=False)
+            X, Y = X[keep], Y[keep]
+        t0 = time.time()
+        model = fit(X, Y, epochs=a.epochs)
+        print(f"[controller] trained on {len(X)} samples in {time.time() - t0:.1f} s; loss "
+              f"{model['hist'][0]:.4f} -> {model['hist'][-1]:.4f}; validation {model['val']}", flush=True)
+        save(model, a.out, n_samples=len(X), epochs=a.epochs)
+
+
+if __name__ == "__main__":
+    main()

@@ -74,6 +74,8 @@ struct ScState {
     double tau_first = 0.0, err_first = 0.0;
     bool   first_done = false;
     int    pred_m = 0, pred_mD = 0;   // predicted Krylov dimensions (deferred checks)
+    double last_err = 0.0;            // error estimate of the last accepted step
+    unsigned long long rng = 0;       // exploration controller's generator
 };
 
 struct tsr_solver {
@@ -340,8 +342,12 @@ int arnoldi(tsr_solver* s, int which)
             std::vector<double>& Hq = which == 0 ? c.H : c.HD;
             std::vector<double>& csq = which == 0 ? c.cs : c.csW;
             const double* hw = s->pin_h + (size_t)q * (jt - c0) * K.ldh;
-            for (int col = c0; col < jt; ++col)
-                for (int i = 0; i <= col + 1; ++i) Hq[(size_t)col * ldh + i] = hw[(size_t)(col - c0) * K.ldh + i];
+            // column col holds rows 0..col+1; the rows below are zero (Hessenberg)
+            for (int col = c0; col < jt; ++col) {
+                double* hc = &Hq[(size_t)col * ldh];
+                for (int i = 0; i <= col + 1; ++i) hc[i] = hw[(size_t)(col - c0) * K.ldh + i];
+                std::fill(hc + col + 2, hc + ldh, 0.0);
+            }
             std::memcpy(csq.data(), s->pin_cs + (size_t)q * (jt + 1), sizeof(double) * (jt + 1));
             const double* nw = s->pin_nw + (size_t)q * jt;
             const double* wr = s->pin_wr + (size_t)q * jt;
@@ -389,6 +395,11 @@ int features_and_proposal(tsr_solver* s, const std::vector<int>& act);
 // ------------------------------------------------------------------------------------
 // One accepted step for every scenario with t < tend.
 // ------------------------------------------------------------------------------------
+static void fail_scen(const char* what, int q)
+{
+    set_error(std::string(what) + " (scenario " + std::to_string(q) + ")");
+}
+
 static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
 {
     const int fail_code = -1;
@@ -401,7 +412,7 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
         ScState& c = s->sc[q];
         c.failed = false; c.accepted = false; c.rejects = 0; c.fe = 0; c.m = 0; c.mD = 0; c.err = 0.0; c.ctrl = 0;
         if (c.t < tend) { act[q] = 1; ++nact; }
-        if (c.stats.NumSteps >= o.MaxNumSteps) return -1;
+        if (c.stats.NumSteps >= o.MaxNumSteps) { set_error("MaxNumSteps reached"); return -1; }
     }
     if (!nact) return 0;
     s->mask_h = act;
@@ -423,7 +434,7 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
     for (int q = 0; q < S; ++q) {
         if (!act[q]) continue;
         ScState& c = s->sc[q];
-        if (c.ctrl != 1) { c.m_cap = o.MaxKrylov; }
+        if (c.ctrl != 1 && c.ctrl != 3) { c.m_cap = o.MaxKrylov; }
         c.proposal = std::fmin(std::fmax(c.proposal, o.MinStep), o.MaxStep);
         c.tstop = tend;
         c.tau = c.proposal;
@@ -436,11 +447,10 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
         if (!act[q]) continue;
         ScState& c = s->sc[q];
         c.fe++;
-        if (s->bad_h[q]) return -1;
+        if (s->bad_h[q]) { fail_scen("RHS not finite at the step start", q); return -1; }
         c.beta = s->res_h[(size_t)q * 8 + 0];
         c.eps = eps_h[q];
         c.cs[0] = 1.0 / c.beta;
-        std::fill(c.H.begin(), c.H.end(), 0.0);
     }
     // main Arnoldi over scenarios with beta > 0
     for (int q = 0; q < S; ++q) s->mask_h[q] = act[q] && s->sc[q].beta > 0.0;
@@ -448,12 +458,12 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
     for (int q = 0; q < S; ++q) {
         if (!act[q]) continue;
         ScState& c = s->sc[q];
-        if (c.failed) return -1;
+        if (c.failed) { fail_scen("RHS not finite in the Krylov basis", q); return -1; }
         if (c.beta > 0.0 && !c.bd) {
             while (krylov_est_main(c, c.m, c.tau, ldh, s->mcap) > kKrylovTol) {
                 c.tau *= 0.5;
                 c.landing = false;
-                if (c.tau < o.MinStep) return -1;
+                if (c.tau < o.MinStep) { fail_scen("Krylov budget: step below MinStep", q); return -1; }
             }
         }
     }
@@ -473,7 +483,7 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
             if (!(c.phi_m == c.m && c.phi_tau == c.tau)) {
                 Tic tic(dense_acc());
                 c.phi_m = -1;
-                if (phi1_pade(c.phi.data(), c.H.data(), c.tau, c.m, ldh, s->mcap)) return -1;
+                if (phi1_pade(c.phi.data(), c.H.data(), c.tau, c.m, ldh, s->mcap)) { set_error("phipade: singular Pade denominator"); return -1; }
                 c.phi_tau = c.tau;
                 c.phi_m = c.m;
             }
@@ -524,28 +534,26 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
         int anyD = 0;
         for (int q = 0; q < S; ++q) anyD |= dq[q];
         if (anyD) {
-            std::vector<double> cs0(S * 2, 0.0);
             for (int q = 0; q < S; ++q) {
                 ScState& c = s->sc[q];
                 if (!dq[q]) continue;
                 c.csW[0] = 1.0 / c.betaD;
-                std::fill(c.HD.begin(), c.HD.end(), 0.0);
             }
-            // csW[0] and veps for the D basis
-            std::vector<double> csrow((size_t)S * s->KW.ld, 0.0), vep(S, 0.0);
-            HIPCHK(hipMemcpyAsync(csrow.data(), s->KW.cs, sizeof(double) * S * s->KW.ld, hipMemcpyDeviceToHost, s->st));
-            HIPCHK(timed_sync(s));
+            // csW[0] and veps for the D basis (column 0 of every scenario's cs row;
+            // scenarios without a D basis get 0 and take no part)
+            std::vector<double> cs0(S, 0.0), vep(S, 0.0);
             for (int q = 0; q < S; ++q) {
                 if (!dq[q]) continue;
-                csrow[(size_t)q * s->KW.ld] = s->sc[q].csW[0];
+                cs0[q] = s->sc[q].csW[0];
                 vep[q] = s->sc[q].eps * s->sc[q].csW[0];
             }
-            HIPCHK(hipMemcpyAsync(s->KW.cs, csrow.data(), sizeof(double) * S * s->KW.ld, hipMemcpyHostToDevice, s->st));
+            HIPCHK(hipMemcpy2DAsync(s->KW.cs, sizeof(double) * s->KW.ld, cs0.data(), sizeof(double), sizeof(double), S,
+                                    hipMemcpyHostToDevice, s->st));
             HIPCHK(hipMemcpyAsync(s->KW.veps, vep.data(), sizeof(double) * S, hipMemcpyHostToDevice, s->st));
             s->mask_h = dq;
             if (arnoldi(s, 1)) return -1;
             for (int q = 0; q < S; ++q)
-                if (dq[q] && s->sc[q].failed) return -1;
+                if (dq[q] && s->sc[q].failed) { fail_scen("RHS not finite in the error-estimate basis", q); return -1; }
         }
         // e = 2 tau betaD W phi3(tau HD) e1; unew = u2 + e; err = WRMS(e)
         for (int q = 0; q < S; ++q) {
@@ -557,7 +565,7 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
             std::vector<double> ph(3 * (size_t)c.mD);
             {
                 Tic tic(dense_acc());
-                if (phik_e1(ph.data(), c.HD.data(), ldh, c.mD, c.tau, 3)) return -1;
+                if (phik_e1(ph.data(), c.HD.data(), ldh, c.mD, c.tau, 3)) { set_error("phi_3 evaluation failed"); return -1; }
             }
             for (int k = 0; k < c.mD; ++k) { c.z[k] = 2.0 * c.tau * c.betaD * ph[2 * c.mD + k]; cf[k] = c.csW[k] * c.z[k]; }
             s->nk_h[q] = c.mD;
@@ -588,7 +596,7 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
                                                               : 0.5;
             c.tau *= fac;
             c.landing = false;
-            if (c.tau < o.MinStep) return -1;
+            if (c.tau < o.MinStep) { fail_scen("error control: step below MinStep", q); return -1; }
         }
         if (nacc) {
             if (S == 1) {
@@ -602,7 +610,7 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
         }
     }
     for (int q = 0; q < S; ++q)
-        if (pend[q]) return -1;
+        if (pend[q]) { fail_scen("no acceptable step after 200 attempts", q); return -1; }
     // bookkeeping
     for (int q = 0; q < S; ++q) {
         if (!act[q]) continue;
@@ -631,6 +639,7 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
         c.pred_m = c.m;
         c.pred_mD = c.mD;
         c.n_hist = std::min(c.n_hist + 1, 4);
+        c.last_err = c.err;
         c.last_hnorm = hnorm;
         c.last_resid = c.hn;
         if (rec) {
@@ -652,10 +661,11 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
 namespace {
 
 // Controller inputs (16 per scenario, order of TSR_FEAT_COUNT in tsr.h) and, with the
-// learned controller active, its proposal: step = kLearnedSafety * 10^y0, clamped to
-// [1/kLearnedTrust, kLearnedTrust] x the classical proposal; Krylov cap from y1.  A
-// scenario whose network output is not finite keeps the classical proposal (ctrl = 2).
-constexpr double kLearnedSafety = 0.8;
+// learned controller active, its proposal: step = 10^y0, at most the classical proposal
+// (which already carries the error-based growth limit) and at least 1/kLearnedTrust of
+// it; Krylov cap kLearnedCap; y1 (the dimension the step is expected to need) as the
+// window of the deferred stopping-rule checks.  A scenario whose network output is not
+// finite keeps the classical proposal (ctrl = 2).
 constexpr double kLearnedTrust = 8.0;
 
 void build_features(const ScState& c, float* f)
@@ -677,12 +687,47 @@ void build_features(const ScState& c, float* f)
     for (int k = 0; k < 6; ++k) f[10 + k] = (float)c.mstat[k];
 }
 
+// Exploration controller (training-data generation): a random walk of the step size
+// that keeps the Krylov dimension of the last step inside [kExploreMlo, kExploreMhi]
+// (x0.7 above, x1.3 below, a log-uniform factor in [e^-0.3, e^0.3] inside), never
+// above the classical error-based step, with the learned controller's Krylov cap.
+constexpr int kExploreMlo = 3, kExploreMhi = 24;
+
+double explore_uniform(unsigned long long& st)
+{
+    unsigned long long z = (st += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
 int features_and_proposal(tsr_solver* s, const std::vector<int>& act)
 {
     const int S = s->S;
     for (int q = 0; q < S; ++q) {
         s->sc[q].ctrl = 0;
         if (act[q]) build_features(s->sc[q], s->sc[q].feat);
+    }
+    if (s->ctrl_mode == 2) {
+        const exprem_options_struct& o = s->opt;
+        for (int q = 0; q < S; ++q) {
+            if (!act[q]) continue;
+            ScState& c = s->sc[q];
+            if (!c.n_hist) continue;   // first step: classical proposal
+            const double tp = c.hist_tau[0];
+            const double mp = c.hist_m[0];
+            double f;
+            if (mp > kExploreMhi) f = 0.7;
+            else if (mp < kExploreMlo) f = 1.3;
+            else f = std::exp(0.6 * explore_uniform(c.rng) - 0.3);
+            double tl = tp * f;
+            if (c.last_err > 0.0) tl = std::fmin(tl, tp * o.SafetyFac * std::pow(c.last_err, -1.0 / 3.0));
+            c.proposal = tl;
+            c.m_cap = std::min(kLearnedCap, s->mcap);
+            c.ctrl = 3;
+        }
+        return 0;
     }
     if (!(s->ctrl_mode == 1 && s->mlp && mlp_ready(s->mlp))) return 0;
     std::vector<float> feat((size_t)S * 16, 0.0f), out((size_t)S * 2, 0.0f);
@@ -694,11 +739,10 @@ int features_and_proposal(tsr_solver* s, const std::vector<int>& act)
         ScState& c = s->sc[q];
         const double y0 = out[(size_t)q * 2], y1 = out[(size_t)q * 2 + 1];
         if (std::isfinite(y0) && std::isfinite(y1) && std::fabs(y0) < 300.0) {
-            const double tl = kLearnedSafety * std::pow(10.0, y0);
-            c.proposal = std::fmin(std::fmax(tl, c.proposal / kLearnedTrust), c.proposal * kLearnedTrust);
-            const int mh = std::min(std::max((int)std::ceil(std::fmin(std::fmax(y1, 0.0), 1000.0)), 5), hi_cap);
-            c.m_cap = std::min(std::max(mh + std::max(4, mh / 4), std::max(5, s->opt.MinKrylov)), hi_cap);
-            c.pred_m = mh;
+            const double tl = std::pow(10.0, y0);
+            c.proposal = std::fmin(std::fmax(tl, c.proposal / kLearnedTrust), c.proposal);
+            c.m_cap = hi_cap;
+            c.pred_m = std::min(std::max((int)std::lround(std::fmin(std::fmax(y1, 0.0), 1000.0)), 5), hi_cap);
             c.ctrl = 1;
         } else {
             c.ctrl = 2;
@@ -867,7 +911,7 @@ void tsr_solver_reset(tsr_solver* s)
         c.proposal = s->opt.InitStep;
         exprem_stats_init(&c.stats, &s->opt, nullptr);
         for (int k = 0; k < 4; ++k) { c.hist_tau[k] = 0.0; c.hist_m[k] = 0.0; }
-        c.n_hist = 0; c.last_hnorm = 0.0; c.last_resid = 0.0; c.pred_m = 0; c.pred_mD = 0;
+        c.n_hist = 0; c.last_hnorm = 0.0; c.last_resid = 0.0; c.pred_m = 0; c.pred_mD = 0; c.last_err = 0.0;
     }
 }
 
@@ -924,10 +968,34 @@ void tsr_solver_micro_shape(const tsr_solver* s, int* bs, int* wpe) { micro_shap
 
 int tsr_solver_time_gfunc_parts(tsr_solver* s, double* g_dev, const double* u_dev, int reps, double* ms3)
 {
+    return tsr_solver_time_gfunc_mode(s, g_dev, u_dev, 0, reps, ms3);
+}
+
+int tsr_solver_time_gfunc_mode(tsr_solver* s, double* g_dev, const double* u_dev, int mode, int reps, double* ms3)
+{
     const int fail_code = 2;
     if (reps < 1) reps = 1;
+    if (mode < 0 || mode > 2) return 1;
     GArgs a;
     a.u = u_dev; a.v = nullptr; a.veps = nullptr; a.g0 = nullptr; a.ieps = nullptr; a.active = nullptr; a.out = g_dev;
+    double* de = nullptr;
+    if (mode > 0) {
+        if (!s->linrec && tsr_solver_set_path(s, s->path | TSR_PATH_JVLIN)) return 2;
+        a.rec = s->linrec;
+        HIPCHK(launch_gfunc(s->P, a, s->st));            // the expansion point (and g0 = G(u))
+        HIPCHK(hipMemcpyAsync(s->g0, g_dev, sizeof(double) * (size_t)s->N, hipMemcpyDeviceToDevice, s->st));
+        if (mode == 2) {
+            // an FD product around u_dev along scratch[0] (zero-filled: the timing does
+            // not depend on the direction), eps = 1
+            std::vector<double> e(2 * (size_t)s->S, 1.0);
+            if (hipMalloc((void**)&de, sizeof(double) * e.size()) != hipSuccess) { set_error("hipMalloc failed"); return 2; }
+            HIPCHK(hipMemcpyAsync(de, e.data(), sizeof(double) * e.size(), hipMemcpyHostToDevice, s->st));
+            HIPCHK(hipMemsetAsync(s->scratch[0], 0, sizeof(double) * (size_t)s->N, s->st));
+            a.rec = nullptr;
+            a.lin = s->linrec;
+            a.v = s->scratch[0]; a.veps = de; a.g0 = s->g0; a.ieps = de + s->S;
+        }
+    }
     hipEvent_t ev[4];
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     double acc[3] = {0, 0, 0};
@@ -946,6 +1014,7 @@ int tsr_solver_time_gfunc_parts(tsr_solver* s, double* g_dev, const double* u_de
     }
     for (auto& e : ev) hipEventDestroy(e);
     for (int part = 0; part < 3; ++part) ms3[part] = acc[part] / reps;
+    if (de) hipFree(de);
     return 0;
 }
 
@@ -1043,9 +1112,14 @@ int tsr_solver_set_path(tsr_solver* s, int path)
 
 int tsr_solver_set_controller(tsr_solver* s, int mode)
 {
-    if (mode != 0 && mode != 1) return 1;
+    if (mode < 0 || mode > 2) return 1;
     s->ctrl_mode = mode;
     return 0;
+}
+
+void tsr_solver_set_seed(tsr_solver* s, unsigned long long seed)
+{
+    for (int q = 0; q < s->S; ++q) s->sc[q].rng = seed * 0x100000001B3ull + (unsigned long long)q;
 }
 
 int tsr_solver_set_mlp(tsr_solver* s, const float* params, const float* mean, const float* stdv)
