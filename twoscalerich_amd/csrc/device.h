@@ -14,7 +14,6 @@
 namespace tsr {
 
 constexpr int kMicroMaxBS = 1024;     // micro-cell workgroup size limit
-constexpr int kMicroNPT = 4;          // nodes per thread at most (bs >= nn / 4)
 constexpr int kMaxNodeInc = 12;       // macro elements around a node hosting a cell
 constexpr int kVecBS = 256;           // threads per workgroup of the vector kernels
 constexpr int kVecR = 4;              // rows per thread of the vector kernels
@@ -73,7 +72,8 @@ struct DScen {
     const int* ptr;
     const int* inc;
     const unsigned char* dir;
-    double* alpha;                    // [4*ne] scratch
+    double* alpha;                    // [4*ne] scratch: interpolation coefficients per macro element
+    double* hel;                      // [4*ne] scratch: vertex heads per macro element (this G call)
     double* qe;                       // [nv*ne] scratch
     double* bq;                       // [2*nv*ne] scratch
 };

@@ -139,6 +139,8 @@ __global__ __launch_bounds__(256) void k_macro_elem(DBatch B, GArgs a)
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) S.alpha[4 * e + k] = al[k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) S.hel[4 * e + k] = hv[k];
     for (int j = 0; j < nv; ++j) {
         const double x = S.midx[j * ne + e], y = S.midy[j * ne + e];
         double gx, gy, hip;
@@ -244,11 +246,11 @@ __device__ __forceinline__ bool is_dirichlet(const DTmpl& T, int i)
     }
 }
 
-// Launch shape (tuned per problem by tune_micro): workgroup size bs (256..1024, a
-// multiple of 64, each thread owning nodes tid, tid + bs, ...; at most kMicroNPT) and a
+// Launch shape (tuned per problem by tune_micro): workgroup size bs (64..1024, a
+// multiple of 64, each thread owning nodes tid, tid + bs, ...; at most 8 of them) and a
 // register budget WPE (amdgpu_waves_per_eu: 1 = compiler default, 6, 8).  Results do
-// not depend on the shape: every per-node and per-element computation is the same, and
-// the exchange sum uses a fixed 256-way order.
+// not depend on the shape: every per-node computation is the same whichever thread does
+// it, and the exchange sum uses a fixed 256-slot order.
 //
 // MODE 0 evaluates G; MODE 1 also records the expansion point (GArgs::rec); MODE 2 takes
 // capacities and conductivities from the recorded first-order expansion (GArgs::lin).
@@ -259,7 +261,7 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
     __shared__ double s_bc[6];
     __shared__ double s_mel[kMaxNodeInc][12];
     __shared__ int s_ninc;
-    __shared__ double s_red[kMicroMaxBS / 64];
+    __shared__ double s_part[256];
     __shared__ double s_geo[2][22];
     const int cell = blockIdx.x;
     const int s = B.cell_scen[cell];
@@ -272,8 +274,8 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
     double* h = smem;
     double* KA = h + B.nn_max;
     double* KB = KA + B.nn_max;
-    double* q = KB + B.nn_max;
-    unsigned char* soil = reinterpret_cast<unsigned char*>(q + 3 * B.ne_max);
+    double* stl = KB + B.nn_max;      // node storage capacity; then the node's net inflow
+    unsigned char* soil = reinterpret_cast<unsigned char*>(stl + B.nn_max);
     const double ve = a.v ? a.veps[s] : 0.0;
     const long long off = B.cell_off[cell];
     const int host = cell - S.first_cell;
@@ -302,7 +304,7 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
 #pragma unroll
             for (int k = 0; k < 9; ++k) s_mel[tid][k] = S.sf[k * S.ne + e];
 #pragma unroll
-            for (int v = 0; v < 3; ++v) s_mel[tid][9 + v] = macro_head(S, a, ve, S.el[v * S.ne + e]);
+            for (int v = 0; v < 3; ++v) s_mel[tid][9 + v] = S.hel[4 * e + v];   // written by k_macro_elem
         }
     }
     if constexpr (STRUCT) {
@@ -310,7 +312,10 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
     }
     __syncthreads();
 
-    // heads: Dirichlet nodes from the macro field (first nbnd threads), the rest from u
+    // heads: Dirichlet nodes from the macro field (first nbnd threads), the rest from u.
+    // In MODE 2 each thread also applies the recorded expansion to the nodes it loads,
+    // so capacities and conductivities are ready with the heads (one phase, one barrier).
+    const size_t NN = (size_t)B.N;
     const int nbnd = STRUCT ? 4 * (T.r - 1) : T.nbnd;
     for (int b = tid; b < nbnd; b += bs) {
         const int i = dirichlet_node<STRUCT>(T, b);
@@ -347,144 +352,154 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
             hi = s_bc[0] * X + s_bc[1] * Y + s_bc[2] * X * Y + s_bc[3];
         }
         h[i] = hi;
+        if constexpr (MODE == 2) {
+            const double* L = a.lin + off + i;
+            const double d = hi - L[0];
+            KA[i] = L[3 * NN] + L[4 * NN] * d;
+            KB[i] = L[5 * NN] + L[6 * NN] * d;
+        }
     }
-    for (int i = tid; i < nn; i += bs)
-        if (!is_dirichlet<STRUCT>(T, i)) h[i] = ld_state(a, ve, off + i);
+    for (int i = tid; i < nn; i += bs) {
+        if (is_dirichlet<STRUCT>(T, i)) continue;
+        const double hi = ld_state(a, ve, off + i);
+        h[i] = hi;
+        if constexpr (MODE == 2) {
+            const double* L = a.lin + off + i;
+            const double d = hi - L[0];
+            stl[i] = L[NN] + L[2 * NN] * d;
+            KA[i] = L[3 * NN] + L[4 * NN] * d;
+            KB[i] = L[5 * NN] + L[6 * NN] * d;
+        }
+    }
     for (int e = tid; e < ne; e += bs) soil[e] = (unsigned char)elem_soil(B, T, cell, e);
     __syncthreads();
 
-    // node-centric constitutive evaluation; storage kept in registers for the gather
-    double stor[kMicroNPT];
-    const size_t NN = (size_t)B.N;
-#pragma unroll
-    for (int k = 0; k < kMicroNPT; ++k) {
-        stor[k] = 1.0;
-        const int i = tid + k * bs;
-        if constexpr (MODE == 2) {
-            if (i < nn) {
-                const double* L = a.lin + off + i;
-                const double d = h[i] - L[0];
-                stor[k] = L[NN] + L[2 * NN] * d;
-                KA[i] = L[3 * NN] + L[4 * NN] * d;
-                KB[i] = L[5 * NN] + L[6 * NN] * d;
-            }
-            continue;
-        }
-        if (i < nn) {
-            double cva = 0.0, cvb = 0.0;
-            for_incident<STRUCT>(T, i, [&](int e, int) {
-                const double sc = STRUCT ? s_geo[e & 1][21] : T.scv[e];
-                if (soil[e] == TSR_SOIL_A) cva += sc; else cvb += sc;
-            });
-            double cA = 0.0, cB = 0.0, kA = 0.0, kB = 0.0, dcA = 0.0, dcB = 0.0, dkA = 0.0, dkB = 0.0;
-            // first evaluation with the lane's own soil (no divergence), second only at interfaces
-            const bool hasA = cva > 0.0, hasB = cvb > 0.0;
+    // node-centric constitutive evaluation (MODE 0/1)
+    if constexpr (MODE != 2) {
+        for (int i = tid; i < nn; i += bs) {
             {
-                DSoil sl;
-                sl.Ks = hasA ? soilA.Ks : soilB.Ks; sl.tr = hasA ? soilA.tr : soilB.tr;
-                sl.ts = hasA ? soilA.ts : soilB.ts; sl.al = hasA ? soilA.al : soilB.al;
-                sl.n = hasA ? soilA.n : soilB.n; sl.m = hasA ? soilA.m : soilB.m;
-                double C, K, dC = 0.0, dK = 0.0;
-                if constexpr (MODE == 1) vg_ckd(sl, h[i], B.Ss, C, K, dC, dK);
-                else vg_ck(sl, h[i], B.Ss, C, K);
-                if (hasA) { cA = C; kA = K; dcA = dC; dkA = dK; } else { cB = C; kB = K; dcB = dC; dkB = dK; }
-            }
-            if (hasA && hasB) {
-                double C, K, dC = 0.0, dK = 0.0;
-                if constexpr (MODE == 1) vg_ckd(soilB, h[i], B.Ss, C, K, dC, dK);
-                else vg_ck(soilB, h[i], B.Ss, C, K);
-                cB = C; kB = K; dcB = dC; dkB = dK;
-            }
-            KA[i] = kA;
-            KB[i] = kB;
-            stor[k] = cA * cva + cB * cvb;
-            if constexpr (MODE == 1) {
-                double* R = a.rec + off + i;
-                R[0] = h[i];
-                R[NN] = stor[k];
-                R[2 * NN] = dcA * cva + dcB * cvb;
-                R[3 * NN] = kA;
-                R[4 * NN] = dkA;
-                R[5 * NN] = kB;
-                R[6 * NN] = dkB;
+                double cva = 0.0, cvb = 0.0;
+                for_incident<STRUCT>(T, i, [&](int e, int) {
+                    const double sc = STRUCT ? s_geo[e & 1][21] : T.scv[e];
+                    if (soil[e] == TSR_SOIL_A) cva += sc; else cvb += sc;
+                });
+                double cA = 0.0, cB = 0.0, kA = 0.0, kB = 0.0, dcA = 0.0, dcB = 0.0, dkA = 0.0, dkB = 0.0;
+                // first evaluation with the lane's own soil (no divergence), second only at interfaces
+                const bool hasA = cva > 0.0, hasB = cvb > 0.0;
+                {
+                    DSoil sl;
+                    sl.Ks = hasA ? soilA.Ks : soilB.Ks; sl.tr = hasA ? soilA.tr : soilB.tr;
+                    sl.ts = hasA ? soilA.ts : soilB.ts; sl.al = hasA ? soilA.al : soilB.al;
+                    sl.n = hasA ? soilA.n : soilB.n; sl.m = hasA ? soilA.m : soilB.m;
+                    double C, K, dC = 0.0, dK = 0.0;
+                    if constexpr (MODE == 1) vg_ckd(sl, h[i], B.Ss, C, K, dC, dK);
+                    else vg_ck(sl, h[i], B.Ss, C, K);
+                    if (hasA) { cA = C; kA = K; dcA = dC; dkA = dK; } else { cB = C; kB = K; dcB = dC; dkB = dK; }
+                }
+                if (hasA && hasB) {
+                    double C, K, dC = 0.0, dK = 0.0;
+                    if constexpr (MODE == 1) vg_ckd(soilB, h[i], B.Ss, C, K, dC, dK);
+                    else vg_ck(soilB, h[i], B.Ss, C, K);
+                    cB = C; kB = K; dcB = dC; dkB = dK;
+                }
+                KA[i] = kA;
+                KB[i] = kB;
+                const double st = cA * cva + cB * cvb;
+                stl[i] = st;
+                if constexpr (MODE == 1) {
+                    double* R = a.rec + off + i;
+                    R[0] = h[i];
+                    R[NN] = st;
+                    R[2 * NN] = dcA * cva + dcB * cvb;
+                    R[3 * NN] = kA;
+                    R[4 * NN] = dkA;
+                    R[5 * NN] = kB;
+                    R[6 * NN] = dkB;
+                }
             }
         }
+        __syncthreads();
     }
-    __syncthreads();
 
-    // sub-face fluxes: q[e*3+j] leaves v_j, enters v_{j+1}
-    for (int e = tid; e < ne; e += bs) {
-        int v0, v1, v2;
-        const double* g;   // structured: per-type geometry
-        if constexpr (STRUCT) {
-            const int sq = e >> 1, r = T.r;
-            const int row = div_rm(T, sq);
-            const int a0 = row * r + (sq - row * (r - 1));
-            v0 = a0;
-            v1 = (e & 1) ? a0 + r + 1 : a0 + 1;
-            v2 = (e & 1) ? a0 + r : a0 + r + 1;
-            g = s_geo[e & 1];
-        } else {
-            v0 = T.el[e]; v1 = T.el[ne + e]; v2 = T.el[2 * ne + e];
-            g = nullptr;
-        }
-        const double h0 = h[v0], h1 = h[v1], h2 = h[v2];
-        double gx, gy;
-        if constexpr (STRUCT) {
-            gx = g[0] * h0 + g[1] * h1 + g[2] * h2;
-            gy = g[3] * h0 + g[4] * h1 + g[5] * h2;
-        } else {
-            gx = T.sf[e] * h0 + T.sf[ne + e] * h1 + T.sf[2 * ne + e] * h2;
-            gy = T.sf[3 * ne + e] * h0 + T.sf[4 * ne + e] * h1 + T.sf[5 * ne + e] * h2;
-        }
-        const double* Ks = (soil[e] == TSR_SOIL_A) ? KA : KB;
-        const double k0 = Ks[v0], k1 = Ks[v1], k2 = Ks[v2];
+    // node-centric fluxes and gather: each interior node visits its incident (element,
+    // local vertex) pairs in ascending element order and evaluates the two sub-face
+    // fluxes it needs, q_j = -K_j (grad h . n_j + n_j,y) (q_j leaves v_j, enters
+    // v_{j+1}): net = sum q_{(lv+2)%3} - q_{lv}.  Each sub-face flux is the same
+    // expression whichever node evaluates it, so the result does not depend on who does.
+    for (int i = tid; i < nn; i += bs) {
+        {
+            double gi = 0.0, net = 0.0;
+            if (!is_dirichlet<STRUCT>(T, i)) {
+                for_incident<STRUCT>(T, i, [&](int e, int lv) {
+                    int v0, v1, v2;
+                    if constexpr (STRUCT) {
+                        const int sq = e >> 1, r = T.r;
+                        const int row = div_rm(T, sq);
+                        const int a0 = row * r + (sq - row * (r - 1));
+                        v0 = a0;
+                        v1 = (e & 1) ? a0 + r + 1 : a0 + 1;
+                        v2 = (e & 1) ? a0 + r : a0 + r + 1;
+                    } else {
+                        v0 = T.el[e]; v1 = T.el[ne + e]; v2 = T.el[2 * ne + e];
+                    }
+                    const double h0 = h[v0], h1 = h[v1], h2 = h[v2];
+                    double gx, gy;
+                    if constexpr (STRUCT) {
+                        const double* g = s_geo[e & 1];
+                        gx = g[0] * h0 + g[1] * h1 + g[2] * h2;
+                        gy = g[3] * h0 + g[4] * h1 + g[5] * h2;
+                    } else {
+                        gx = T.sf[e] * h0 + T.sf[ne + e] * h1 + T.sf[2 * ne + e] * h2;
+                        gy = T.sf[3 * ne + e] * h0 + T.sf[4 * ne + e] * h1 + T.sf[5 * ne + e] * h2;
+                    }
+                    const double* Ks = (soil[e] == TSR_SOIL_A) ? KA : KB;
+                    const double k0 = Ks[v0], k1 = Ks[v1], k2 = Ks[v2];
+                    const int jin = lv == 0 ? 2 : lv - 1;   // (lv + 2) % 3: the sub-face entering v_lv
+                    double qj[2];
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            double Kf, nx, ny;
-            if constexpr (STRUCT) {
-                Kf = g[12 + j * 3] * k0 + g[12 + j * 3 + 1] * k1 + g[12 + j * 3 + 2] * k2;
-                nx = g[6 + 2 * j];
-                ny = g[6 + 2 * j + 1];
-            } else {
-                Kf = T.w[(j * 3) * ne + e] * k0 + T.w[(j * 3 + 1) * ne + e] * k1 + T.w[(j * 3 + 2) * ne + e] * k2;
-                nx = T.nrm[(2 * j) * ne + e];
-                ny = T.nrm[(2 * j + 1) * ne + e];
+                    for (int t2 = 0; t2 < 2; ++t2) {
+                        const int j = t2 == 0 ? jin : lv;
+                        double Kf, nx, ny;
+                        if constexpr (STRUCT) {
+                            const double* g = s_geo[e & 1];
+                            Kf = g[12 + j * 3] * k0 + g[12 + j * 3 + 1] * k1 + g[12 + j * 3 + 2] * k2;
+                            nx = g[6 + 2 * j];
+                            ny = g[6 + 2 * j + 1];
+                        } else {
+                            Kf = T.w[(j * 3) * ne + e] * k0 + T.w[(j * 3 + 1) * ne + e] * k1 + T.w[(j * 3 + 2) * ne + e] * k2;
+                            nx = T.nrm[(2 * j) * ne + e];
+                            ny = T.nrm[(2 * j + 1) * ne + e];
+                        }
+                        qj[t2] = -Kf * (gx * nx + (gy + 1.0) * ny);
+                    }
+                    net += qj[0];
+                    net -= qj[1];
+                    // one incident pair at a time: keeps the structured visitor's six
+                    // straight-line visits from hoisting every LDS load (register spills)
+                    __builtin_amdgcn_sched_barrier(0);
+                });
+                gi = net / stl[i];
             }
-            q[e * 3 + j] = -Kf * (gx * nx + (gy + 1.0) * ny);
-        }
-    }
-    __syncthreads();
-
-    // node-centric gather (incident order = element order, identical on every run);
-    // interior net inflows go to LDS (KA is free now) for the exchange sum
-    double* netl = KA;
-#pragma unroll
-    for (int k = 0; k < kMicroNPT; ++k) {
-        const int i = tid + k * bs;
-        if (i < nn) {
-            double net = 0.0;
-            for_incident<STRUCT>(T, i, [&](int e, int lv) {
-                net += q[e * 3 + (lv + 2) % 3];
-                net -= q[e * 3 + lv];
-            });
-            double gi;
-            if (is_dirichlet<STRUCT>(T, i)) { gi = 0.0; netl[i] = 0.0; }
-            else { gi = net / stor[k]; netl[i] = net; }
+            stl[i] = net;   // only this thread reads or writes entry i in this phase
             store_out(B, a, s, off + i, gi);
         }
     }
     __syncthreads();
-    // exchange Q = sum of interior net inflows in a fixed order (threads 0..255 take
-    // nodes t, t+256, ...; then the 4 wavefronts in order), independent of bs
-    if (tid < 256) {
+    // exchange Q = sum of interior net inflows in a fixed order, independent of bs:
+    // 256 slots t take nodes t, t+256, ... in order; the first wavefront then sums slots
+    // w*64 + lane with the shuffle tree for w = 0..3 and adds the four in a fixed tree.
+    for (int t2 = tid; t2 < 256; t2 += bs) {
         double Qacc = 0.0;
-        for (int i = tid; i < nn; i += 256) Qacc += netl[i];
-        Qacc = wave_sum(Qacc);
-        if ((tid & 63) == 0) s_red[tid >> 6] = Qacc;
+        for (int i = t2; i < nn; i += 256) Qacc += stl[i];
+        s_part[t2] = Qacc;
     }
     __syncthreads();
-    if (tid == 0) B.Q[cell] = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+    if (tid < 64) {
+        double r4[4];
+#pragma unroll
+        for (int w = 0; w < 4; ++w) r4[w] = wave_sum(s_part[w * 64 + tid]);
+        if (tid == 0) B.Q[cell] = (r4[0] + r4[1]) + (r4[2] + r4[3]);
+    }
 }
 
 // ---------------------------------------------------------------- macro nodes
@@ -644,7 +659,7 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
     Arena A;
     struct TOff { size_t ptr, inc, el, sf, nrm, w, scv, vec, cen, bnd, soil, blist; int nn, ne; double ca; bool has_soil;
                   int structured, r, nbnd; unsigned mul_r, mul_rm; double sg[2][22]; };
-    struct SOff { size_t keff, el, nodes, sf, nrm, midx, midy, cv, scv, bce, blen, bmid, bnrm, cen, ptr, inc, dir, alpha, qe, bq; };
+    struct SOff { size_t keff, el, nodes, sf, nrm, midx, midy, cv, scv, bce, blen, bmid, bnrm, cen, ptr, inc, dir, alpha, hel, qe, bq; };
     std::vector<TOff> to(v.n_tmpl);
     std::vector<SOff> so(v.n_scen);
     int nn_max = 0, ne_max = 0, mnn = 0, mne = 0;
@@ -652,7 +667,7 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
         const micro_mesh_struct& m = v.tmpl[t];
         if (m.no_verts != 3) { err = "micro templates must be triangles"; return nullptr; }
         const int nn = m.no_nodes, ne = m.no_elements;
-        if (nn > kMicroMaxBS * kMicroNPT) { err = "micro template has more than 4096 nodes"; return nullptr; }
+        if (nn > kMicroMaxBS * 8) { err = "micro template has more than 8192 nodes"; return nullptr; }
         nn_max = std::max(nn_max, nn);
         ne_max = std::max(ne_max, ne);
         std::vector<int> ptr, inc;
@@ -738,6 +753,7 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
             o.keff = keep;
         }
         o.alpha = A.reserve<double>((size_t)4 * ne);
+        o.hel = A.reserve<double>((size_t)4 * ne);
         o.qe = A.reserve<double>((size_t)nv * ne);
         o.bq = A.reserve<double>((size_t)2 * nv * ne);
     }
@@ -802,7 +818,7 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
         S.be_len = at<double>(base, o.blen); S.be_mid = at<double>(base, o.bmid); S.be_nrm = at<double>(base, o.bnrm);
         S.cen = at<double>(base, o.cen); S.ptr = at<int>(base, o.ptr); S.inc = at<int>(base, o.inc);
         S.dir = at<unsigned char>(base, o.dir);
-        S.alpha = at<double>(base, o.alpha); S.qe = at<double>(base, o.qe); S.bq = at<double>(base, o.bq);
+        S.alpha = at<double>(base, o.alpha); S.hel = at<double>(base, o.hel); S.qe = at<double>(base, o.qe); S.bq = at<double>(base, o.bq);
         std::memcpy(A.host.data() + o_S + s * sizeof(DScen), &S, sizeof(DScen));
     }
     if (hipMemcpy(P->buf, A.host.data(), P->bytes, hipMemcpyHostToDevice) != hipSuccess) {
@@ -821,7 +837,7 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
     B.Ss = v.Ss; B.N = v.N;
     P->all_structured = !std::getenv("TSR_GENERIC_MICRO");
     for (int t = 0; t < v.n_tmpl; ++t) P->all_structured = P->all_structured && to[t].structured;
-    P->lds = ((size_t)3 * nn_max + (size_t)3 * ne_max) * sizeof(double) + (size_t)ne_max;
+    P->lds = (size_t)4 * nn_max * sizeof(double) + (size_t)ne_max;
     P->lds = (P->lds + 15) & ~size_t(15);
     if (P->lds > 64 * 1024) {
         bool ok = true;
@@ -888,12 +904,12 @@ static hipError_t tune_micro(DevProblem* p, const GArgs& a, hipStream_t st)
 {
     const DBatch& B = p->B;
     const int mode = micro_mode(a);
-    const int bs_min = std::max(256, (B.nn_max + kMicroNPT - 1) / kMicroNPT);
     std::vector<int> cand_bs;
-    for (int bs : {256, 320, 384, 448, 512, 640, 768, 1024}) {
-        const int b = (std::max(bs, bs_min) + 63) / 64 * 64;
-        if (b <= kMicroMaxBS && std::find(cand_bs.begin(), cand_bs.end(), b) == cand_bs.end()) cand_bs.push_back(b);
+    for (int bs : {64, 128, 192, 256, 384, 512, 768, 1024}) {
+        // at most 8 strided nodes per thread
+        if (bs <= kMicroMaxBS && bs * 8 >= B.nn_max) cand_bs.push_back(bs);
     }
+    if (cand_bs.empty()) cand_bs.push_back(kMicroMaxBS);
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return hipErrorUnknown;
     double best = 1e300;
@@ -924,15 +940,17 @@ hipError_t launch_gfunc_part(const DevProblem* p, const GArgs& a, int part, hipS
 {
     const DBatch& B = p->B;
     if (part == 0) {
-        dim3 ge((B.macro_ne_max + 255) / 256, B.n_scen);
-        hipLaunchKernelGGL(k_macro_elem, ge, dim3(256), 0, st, B, a);
+        // 64-thread workgroups: the macro meshes are small (10^4-10^5 elements), so
+        // spread them over as many CUs as possible
+        dim3 ge((B.macro_ne_max + 63) / 64, B.n_scen);
+        hipLaunchKernelGGL(k_macro_elem, ge, dim3(64), 0, st, B, a);
     } else if (part == 1) {
         const int mode = micro_mode(a);
         if (p->micro_bs[mode] == 0) return tune_micro(const_cast<DevProblem*>(p), a, st);
         launch_micro(p, a, p->micro_bs[mode], p->micro_wpe[mode], st);
     } else {
-        dim3 gn((B.macro_nn_max + 255) / 256, B.n_scen);
-        hipLaunchKernelGGL(k_macro_node, gn, dim3(256), 0, st, B, a);
+        dim3 gn((B.macro_nn_max + 63) / 64, B.n_scen);
+        hipLaunchKernelGGL(k_macro_node, gn, dim3(64), 0, st, B, a);
     }
     return hipGetLastError();
 }
