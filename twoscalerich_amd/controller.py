@@ -27,7 +27,8 @@ scenario within +-`window` steps (the state changes little over that span):
     mD(tau_c) are the medians over records with |log10 tau - log10 tau_c| <= 0.05; the
     cost of a step is (in units of one linearised RHS evaluation)
         cost(m, mD) = c0 + m + mD + r (m^2 + 8 m + mD^2 + 5 mD)
-    (RHS evaluations, plus r per basis column read in the Gram-Schmidt passes); the
+    (RHS evaluations, plus r per basis column read in the Gram-Schmidt passes and the
+    combinations; c0 = 4.7, r = 0.037 from the measured kernel times); the
     label is the tau_c with the largest tau_c / cost, among candidates whose first
     attempt's error was at most `err_max`:
         y0 = log10 tau_c,  y1 = m(tau_c).
@@ -70,8 +71,9 @@ def records_to_arrays(recs):
 
 
 # cost model of one step on the batched path, in units of one linearised RHS evaluation
-COST_C0 = 5.5      # G(u) with the expansion record, G(U2), the combinations, host syncs
-COST_R = 0.064     # one basis-column read in a Gram-Schmidt pass / one RHS evaluation
+# (plain workload on MI355X: 0.195 ms per linearised G, 7.2 us per basis-column read)
+COST_C0 = 4.7      # G(u) with the expansion record, G(U2), host syncs, controller
+COST_R = 0.037     # one basis-column read in a Gram-Schmidt pass / one RHS evaluation
 
 
 def step_cost(m, mD, c0=COST_C0, r=COST_R):
