@@ -1,28 +1,37 @@
 """Benchmark of the TwoScaleRich time-stepping hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--controller classical|learned]
-                    [--workload plain] [--tend T] [--cpu-sample-s S] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--plain-tend T] [--full-plain]
+                    [--cpu-sample-s S] [--no-cpu]
 
 Metric (the reference's headline for this path): wall-time to t_end of the exponential
-integrator on the plain GPU-scale workload (macro 101x101 nodes, 10,201 micro cells of
-21x21 nodes, N = 4,508,842 FP64 unknowns, t in [0, 1] day, rtol = atol = 1e-6, Krylov
-dimension <= 30), with the RHS throughput in GB/s beside it.
+integrator, learned + batched GPU EXPREM against plain GPU EXPREM, on the plain GPU-scale
+workload (macro 101x101 nodes, 10,201 micro cells of 21x21 nodes, N = 4,508,842 FP64
+unknowns, t in [0, 1] day, rtol = atol = 1e-6), with the RHS throughput beside it.
 
 A "step" is one pass of the hot path over the workload: the integration from t = 0 to
-t_end (every accepted EXPREM step, all on the device; inputs resident in HBM before the
-timer starts).  `value` is its wall-time in seconds (lower is better).  W warm-up steps
-integrate a short horizon first (kernel loading, allocation); K timed steps each
-integrate the full horizon from the same initial state.
+t_end with the learned controller on the batched solver path (every accepted EXPREM
+step, all on the device; inputs resident in HBM before the timer starts).  `value` is
+its wall-time in seconds (lower is better).  W warm-up steps integrate a short horizon
+first (kernel loading, launch-shape tuning); K timed steps each integrate the full
+horizon from the same initial state.
 
 Extra fields on the JSON line:
-  roofline      the dominant kernel (k_micro, one workgroup per micro cell):
-                achieved = algorithmic bytes of one launch (16 B per micro unknown: read
-                u, write g, plus per-cell distinct data) / its average duration from HIP
-                events on the solver stream; peak 8 TB/s; traffic = FETCH+WRITE bytes per
-                launch from rocprofv3 counters (profiles/), or null.
-  cpu_baseline  the CPU oracle (FP64 C++/OpenMP restatement of the same method, "port")
-                timed on this host: EXPREM steps of the same workload for a bounded
-                sample, scaled to the GPU run's step count (labelled in "sample").
+  plain         the plain GPU path (the reference algorithm: classical controller, CGS2,
+                FD products of the full RHS, a host check per Arnoldi iteration) timed to
+                --plain-tend in this run, the learned+batched run's time to the same point
+                and the solutions' difference there; with --full-plain the plain path is
+                also timed to t_end (about 5 minutes).  `full_committed` repeats the
+                plain path's full-horizon time from profiles/ for reference.
+  gfunc         G evaluation times per kernel for the three micro-kernel modes.
+  roofline      the dominant kernel of the timed run (k_micro in its linearised mode, the
+                FD products): algorithmic bytes of one launch (88 B per interior micro
+                node: u, v, 7 expansion components, g0, out; 56 B per Dirichlet node) over
+                its average duration from HIP events on the solver stream; peak 8 TB/s;
+                traffic = FETCH+WRITE bytes per launch from rocprofv3 counters (profiles/),
+                scaled by the copy-kernel calibration, or null.
+  cpu_baseline  the CPU oracle (FP64 C++/OpenMP restatement of the plain algorithm,
+                "port") timed on this host: EXPREM steps of the same workload for a
+                bounded sample, scaled to t_end by the plain GPU path's time profile.
 """
 import argparse
 import ctypes as C
@@ -37,7 +46,39 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_micro_plain_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_micro_lin_plain_traffic.json")
+PLAIN_FULL_FILE = os.path.join(ROOT, "profiles", "exprem_plain_full.json")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def integrate(T, s, u0, tend, chunk, label, stops=()):
+    """Run solver s from (u0, 0) to tend; returns (wall, totals, {stop: wall at stop})."""
+    L = T.lib()
+    s.reset()
+    s.set_state(u0, 0.0)
+    L.tsr_device_sync()
+    t0 = time.perf_counter()
+    tot = {"steps": 0, "rejects": 0, "fevals": 0, "host_dense_s": 0.0, "sync_wait_s": 0.0, "t": 0.0}
+    at = {}
+    for target in [x for x in stops if x < tend] + [tend]:
+        while True:
+            rc, st, _ = s.run(target, chunk)
+            for key in ("steps", "rejects", "fevals", "host_dense_s", "sync_wait_s"):
+                tot[key] += getattr(st, key)
+            tot["t"] = st.t
+            log(f"{label}: t={st.t:.4e}/{tend} exprem_steps={tot['steps']} wall={time.perf_counter() - t0:.1f}s")
+            if rc:
+                raise RuntimeError(f"{label}: integration stopped at t={st.t} rc={rc}: " + T.last_error())
+            if st.t >= target:
+                break
+        if target != tend:
+            L.tsr_device_sync()
+            at[target] = (time.perf_counter() - t0, s.get_state()[0])
+    L.tsr_device_sync()
+    return time.perf_counter() - t0, tot, at
 
 
 def main():
@@ -45,88 +86,97 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel timing section")
-    ap.add_argument("--controller", default="classical", choices=["classical", "learned"])
-    ap.add_argument("--workload", default="plain", choices=["plain"])
-    ap.add_argument("--tend", type=float, default=0.0, help="override t_end (reported in config)")
     ap.add_argument("--warmup-tend", type=float, default=1e-6)
+    ap.add_argument("--plain-tend", type=float, default=0.25, help="horizon of the plain-path comparison run")
+    ap.add_argument("--tend", type=float, default=0.0, help="override t_end (reported in config; default the workload's)")
+    ap.add_argument("--full-plain", action="store_true", help="also time the plain path to t_end")
     ap.add_argument("--cpu-sample-s", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
-    ap.add_argument("--max-steps", type=int, default=10 ** 7)
-    ap.add_argument("--chunk", type=int, default=500, help="EXPREM steps between progress lines")
+    ap.add_argument("--chunk", type=int, default=1000, help="EXPREM steps between progress lines")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU; the path's sharding is described in DESIGN.md "
-              "(--gpus must be 1)", file=sys.stderr)
+        print("bench.py: this build runs on one GPU (--gpus must be 1); the path's sharding across GPUs is "
+              "described in the design notes, not built", file=sys.stderr)
         sys.exit(2)
 
     import twoscalerich_amd as T
     from twoscalerich_amd import _ffi as F
+    from twoscalerich_amd import controller as CT
     L = T.lib()
 
     p = T.Problem.config(F.CFG_PLAIN)
     opt = p.options()
     tend = args.tend if args.tend > 0 else p.tend()
-    s = T.Solver(p, opt)
-    if args.controller == "learned":
-        from twoscalerich_amd import controller as CT
-        CT.attach(s)
+    args.plain_tend = min(args.plain_tend, tend)
     u0 = p.initial()
 
-    # warm-up: short horizon from the same initial state
-    for _ in range(args.warmup):
-        s.reset()
-        s.set_state(u0, 0.0)
-        rc, st, _ = s.run(args.warmup_tend, args.max_steps)
-        if rc:
-            raise RuntimeError("warm-up failed: " + T.last_error())
-    L.tsr_device_sync()
+    fast = T.Solver(p, opt)
+    fast.set_path(F.PATH_BATCHED)
+    CT.attach(fast)
+    plain = T.Solver(p, opt)     # plain path, classical controller (the defaults)
 
-    walls, runs = [], []
+    for _ in range(args.warmup):
+        for s, lab in ((fast, "warm-up learned+batched"), (plain, "warm-up plain")):
+            integrate(T, s, u0, args.warmup_tend, args.chunk, lab)
+
+    # timed: learned + batched to t_end (the metric), noting the wall time at plain_tend
+    walls, runs, at_prefix = [], [], None
     for k in range(args.steps):
-        s.reset()
-        s.set_state(u0, 0.0)
-        L.tsr_device_sync()
-        t0 = time.perf_counter()
-        tot = {"steps": 0, "rejects": 0, "fevals": 0, "host_dense_s": 0.0, "sync_wait_s": 0.0, "t": 0.0}
-        while True:
-            rc, st, _ = s.run(tend, min(args.chunk, args.max_steps - tot["steps"]))
-            for key in ("steps", "rejects", "fevals", "host_dense_s", "sync_wait_s"):
-                tot[key] += getattr(st, key)
-            tot["t"] = st.t
-            print(f"[bench] step {k}: t={st.t:.4e}/{tend} exprem_steps={tot['steps']} "
-                  f"wall={time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
-            if rc or st.t >= tend or tot["steps"] >= args.max_steps:
-                break
-        L.tsr_device_sync()
-        walls.append(time.perf_counter() - t0)
+        w, tot, at = integrate(T, fast, u0, tend, args.chunk, f"learned+batched step {k}", stops=(args.plain_tend,))
+        walls.append(w)
         runs.append(tot)
-        if rc or tot["t"] < tend:
-            raise RuntimeError(f"integration stopped at t={tot['t']} rc={rc}: " + T.last_error())
+        at_prefix = at.get(args.plain_tend, (w, fast.get_state()[0]))
     wall = max(walls)
     st = runs[-1]
-    u_end, _ = s.get_state()
+    u_end, _ = fast.get_state()
 
-    # RHS throughput and the dominant kernel's roofline (HIP events on the solver stream)
-    du = L.tsr_solver_dev_u(s.h)
-    g = L.tsr_solver_dev_scratch(s.h, 1)
-    L.tsr_solver_gfunc_dev(s.h, g, du, 3)
-    L.tsr_solver_gfunc_dev(s.h, g, du, args.kernel_reps)
-    g_ms = L.tsr_solver_last_gfunc_ms(s.h)
-    ms3 = (C.c_double * 3)()
-    L.tsr_solver_time_gfunc_parts(s.h, g, du, args.kernel_reps, ms3)
+    # plain path to plain_tend (and optionally to t_end)
+    w_plain, tot_plain, _ = integrate(T, plain, u0, args.plain_tend, args.chunk, "plain")
+    u_plain, _ = plain.get_state()
+    d2 = float(np.linalg.norm(at_prefix[1] - u_plain) / np.linalg.norm(u_plain))
+    dinf = float(np.max(np.abs(at_prefix[1] - u_plain)) / np.max(np.abs(u_plain)))
+    plain_info = {"t_end_days": args.plain_tend, "wall_s": round(w_plain, 3), "steps": tot_plain["steps"],
+                  "rhs_evals": tot_plain["fevals"], "learned_batched_wall_s": round(at_prefix[0], 3),
+                  "speedup": round(w_plain / at_prefix[0], 3),
+                  "rel_diff_2norm": d2, "rel_diff_max": dinf}
+    if args.full_plain:
+        w_full, tot_full, _ = integrate(T, plain, u0, tend, args.chunk, "plain full")
+        u_pf, _ = plain.get_state()
+        plain_info["full"] = {"wall_s": round(w_full, 3), "steps": tot_full["steps"], "rhs_evals": tot_full["fevals"],
+                              "speedup": round(w_full / wall, 3),
+                              "rel_diff_2norm_t_end": float(np.linalg.norm(u_end - u_pf) / np.linalg.norm(u_pf)),
+                              "rel_diff_max_t_end": float(np.max(np.abs(u_end - u_pf)) / np.max(np.abs(u_pf)))}
+    if os.path.exists(PLAIN_FULL_FILE):
+        with open(PLAIN_FULL_FILE) as f:
+            ref = json.load(f)
+        plain_info["full_committed"] = {"wall_s": ref["wall_s"], "speedup": round(ref["wall_s"] / wall, 3),
+                                        "source": os.path.relpath(PLAIN_FULL_FILE, ROOT)}
+
+    # G kernels per mode and the dominant kernel's roofline (HIP events on the solver stream)
+    du = L.tsr_solver_dev_u(fast.h)
+    g = L.tsr_solver_dev_scratch(fast.h, 1)
+    gf = {}
+    for mode, name in ((0, "full"), (1, "record"), (2, "linearised_fd")):
+        ms3 = (C.c_double * 3)()
+        L.tsr_solver_time_gfunc_mode(fast.h, g, du, mode, 3, ms3)
+        L.tsr_solver_time_gfunc_mode(fast.h, g, du, mode, args.kernel_reps, ms3)
+        gf[name] = {"macro_elem_ms": round(ms3[0], 4), "micro_ms": round(ms3[1], 4), "macro_node_ms": round(ms3[2], 4)}
     v = p.view.contents
     M = v.scen[0].ud.macro_mesh.no_nodes
     n_micro = p.N - M
-    micro_bytes = 16.0 * n_micro + p.distinct_bytes()
-    g_bytes = 16.0 * p.N + p.distinct_bytes()
-    micro_gbs = micro_bytes / (ms3[1] * 1e-3) / 1e9
-    shape = (C.c_int(), C.c_int())
-    L.tsr_solver_micro_shape(s.h, C.byref(shape[0]), C.byref(shape[1]))
+    r = v.tmpl[0].no_nodes
+    n_cells = v.n_cells
+    n_dir = n_cells * (4 * (int(round(r ** 0.5)) - 1))
+    n_int = n_micro - n_dir
+    lin_bytes = 88.0 * n_int + 56.0 * n_dir + p.distinct_bytes()
+    lin_ms = gf["linearised_fd"]["micro_ms"]
+    achieved = lin_bytes / (lin_ms * 1e-3) / 1e9
+    full_bytes = 16.0 * p.N + p.distinct_bytes()
+    gf["full"]["GB_per_s_16N"] = round(full_bytes / (gf["full"]["micro_ms"] * 1e-3) / 1e9, 1)
     copy_ms = C.c_double()
-    L.tsr_solver_time_copy(s.h, args.kernel_reps, C.byref(copy_ms))
+    L.tsr_solver_time_copy(fast.h, args.kernel_reps, C.byref(copy_ms))
     copy_gbs = 16.0 * p.N / (copy_ms.value * 1e-3) / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
@@ -135,57 +185,60 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(p, opt, tend, args.cpu_sample_s)
-        # the GPU's time for the same first n steps, to scale the sample to t_end
-        s.reset()
-        s.set_state(u0, 0.0)
+        cpu = cpu_baseline(T, p, opt, tend, args.cpu_sample_s)
+        # the plain GPU path's time for the same first n steps scales the sample to t_end
+        plain.reset()
+        plain.set_state(u0, 0.0)
         L.tsr_device_sync()
         t0 = time.perf_counter()
-        s.run(tend, cpu["_n"])
+        plain.run(tend, cpu["_n"])
         L.tsr_device_sync()
         gpu_n = time.perf_counter() - t0
-        cpu["value"] = round(cpu["_cpu_s"] * wall / gpu_n, 1)
-        cpu["sample"] += (f"; the GPU took {gpu_n:.3f} s for the same {cpu['_n']} steps and {wall:.1f} s to t_end, "
-                          f"so the CPU time to t_end is estimated as {cpu['_cpu_s']:.1f} s x {wall / gpu_n:.1f}")
-        cpu["speedup_same_steps"] = round(cpu["_cpu_s"] / gpu_n, 1)
+        full_plain = plain_info.get("full", plain_info.get("full_committed", {})).get("wall_s")
+        if full_plain:
+            cpu["value"] = round(cpu["_cpu_s"] * full_plain / gpu_n, 1)
+            cpu["sample"] += (f"; the plain GPU path took {gpu_n:.3f} s for the same {cpu['_n']} steps and "
+                              f"{full_plain:.1f} s to t_end, so the CPU time to t_end is estimated as "
+                              f"{cpu['_cpu_s']:.1f} s x {full_plain / gpu_n:.1f}")
+        cpu["speedup_plain_gpu_same_steps"] = round(cpu["_cpu_s"] / gpu_n, 1)
         del cpu["_n"], cpu["_cpu_s"]
 
     line = {
         "metric": "wall-time to t_end, learned+batched vs plain GPU EXPREM at <=1e-6 rel err; G GB/s",
-        "value": round(wall, 4),
+        "value": round(wall, 3),
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall * 1e3, 2),
+        "ms_per_step": round(wall * 1e3, 1),
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "plain GPU scale: macro 101x101, micro 21x21 per macro node, loam matrix + "
-                               "centred sand disc r=0.25, Dirichlet -0.1 m top, h0=-5 m",
+                               "centred sand disc r=0.25, Dirichlet -0.1 m top, h0=-5 m, t in [0, 1] d",
                    "N": p.N, "t_end_days": tend, "rtol": opt.RelTol, "atol": opt.AbsTol,
-                   "max_krylov": opt.MaxKrylov, "controller": args.controller, "mode": "plain"},
+                   "max_krylov_plain": opt.MaxKrylov, "controller": "learned (Krylov range [5, 60])",
+                   "path": "batched (DCGS2, deferred Krylov checks, linearised FD products)"},
         "exprem": {"accepted_steps": st["steps"], "rejected": st["rejects"], "rhs_evals": st["fevals"],
                    "host_dense_s": round(st["host_dense_s"], 3), "sync_wait_s": round(st["sync_wait_s"], 3),
                    "u_end_min": float(u_end.min()), "u_end_max": float(u_end.max())},
-        "gfunc": {"ms": round(g_ms, 4), "GB_per_s": round(g_bytes / (g_ms * 1e-3) / 1e9, 1),
-                  "kernel_ms": {"macro_elem": round(ms3[0], 4), "micro": round(ms3[1], 4),
-                                "macro_node": round(ms3[2], 4)}},
-        "roofline": {"kernel": "k_micro", "bound": "hbm", "achieved": round(micro_gbs, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(micro_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes": micro_bytes, "avg_launch_ms": round(ms3[1], 4),
-                     "copy_kernel_GB_per_s": round(copy_gbs, 1),
-                     "launch_shape": {"workgroup": shape[0].value, "waves_per_eu": shape[1].value}},
+        "plain": plain_info,
+        "gfunc": gf,
+        "roofline": {"kernel": "k_micro (linearised FD product)", "bound": "hbm", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "algorithmic_bytes": lin_bytes, "avg_launch_ms": lin_ms,
+                     "copy_kernel_GB_per_s": round(copy_gbs, 1)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
-    s.close()
+    fast.close()
+    plain.close()
 
 
-def cpu_baseline(p, opt, tend, budget_s):
-    """The CPU oracle (FP64 C++/OpenMP restatement of the same method) on the same
+def cpu_baseline(T, p, opt, tend, budget_s):
+    """The CPU oracle (FP64 C++/OpenMP restatement of the plain algorithm) on the same
     workload from t = 0, bounded: EXPREM steps until the time budget is spent (at least
     one step)."""
     from twoscalerich_amd import _ffi as F
@@ -204,11 +257,11 @@ def cpu_baseline(p, opt, tend, budget_s):
         if rc:
             break
         n += 1
-        print(f"[bench] cpu oracle step {n}: t={t:.3e} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+        log(f"cpu oracle step {n}: t={t:.3e} ({time.perf_counter() - t0:.1f} s)")
     el = time.perf_counter() - t0
     o.close()
     return {"value": None, "unit": "s", "cores": cores, "kind": "port", "_n": n, "_cpu_s": el,
-            "sample": f"CPU oracle: the first {n} EXPREM steps of the same run from t=0 took {el:.1f} s "
+            "sample": f"CPU oracle: the first {n} EXPREM steps of the plain run from t=0 took {el:.1f} s "
                       f"(FuncEvals {st.FuncEvals}, t reached {t:.3e} d, OpenMP threads {cores})"}
 
 

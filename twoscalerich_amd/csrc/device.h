@@ -42,6 +42,7 @@ struct DTmpl {
     // K weights (9, j*3+v), SCV area (1).
     int structured;
     int r;
+    double dx;                        // grid spacing (structured templates)
     unsigned mul_r, mul_rm;           // n / r == __umulhi(n, mul_r), n / (r-1) == __umulhi(n, mul_rm), n < 2^16
     int nbnd;                         // Dirichlet nodes (generic templates: listed in bnd_list)
     const int* bnd_list;              // [nbnd] (generic templates)
@@ -171,6 +172,11 @@ hipError_t launch_dcgs2(int nk, const double* const* cols, double* w, const doub
                        const int* active, double at, double rt, hipStream_t st);
 hipError_t launch_masked_copy(double* dst, const double* src, const BlkMap& M, int nblk, const int* mask,
                               hipStream_t st);
+// Readbacks into a mapped pinned host buffer (layouts in krylov.hip)
+hipError_t launch_pack_window(const KrylovState& K, int c0, int jt, const int* bad, const int* active, int n_scen,
+                              double* out, int stride, hipStream_t st);
+hipError_t launch_pack_small(const double* res, const int* bad, const double* eps, const double* stats, int n_scen,
+                             double* out, hipStream_t st);
 // out[s*6 + 0..5]: mean, min, max macro head; mean, min, max matrix-soil saturation
 hipError_t launch_macro_stats(const DScen* scen, int n_scen, const double* u, double* out, hipStream_t st);
 

@@ -246,6 +246,73 @@ __device__ __forceinline__ bool is_dirichlet(const DTmpl& T, int i)
     }
 }
 
+// Net inflow of interior node i = (ix, iy) of the uniform right-triangle grid with
+// spacing d (1 <= ix, iy <= r-2: all six incident elements exist).  On this grid the
+// element geometry is exact rationals times d (checked against the template's own
+// geometry by detect_structured):
+//   A = (a, a+1, a+r+1): d grad h = (h1 - h0, h2 - h1); sub-face normals / d =
+//       (1/3, -1/6), (-1/6, 1/3), (-1/6, -1/6)
+//   B = (a, a+r+1, a+r): d grad h = (h1 - h2, h2 - h0); normals / d =
+//       (1/6, 1/6), (-1/3, 1/6), (1/6, -1/3)
+//   sub-face j's K weights (the P1 functions at its midpoint): 5/12 on v_j and v_{j+1},
+//   1/6 on v_{j+2};  q_j = -K_j ((grad h + e_y) . n_j), leaving v_j, entering v_{j+1}.
+// The incident pairs are summed in ascending element order, net += q_in; net -= q_out,
+// as the generic visitor does:
+//   LD.A (2), LD.B (1), RD.B (2), LU.A (1), RU.A (0), RU.B (0)   (local vertex of i)
+template <int TYPE, int J>
+__device__ __forceinline__ double q_struct(double h0, double h1, double h2, double k0, double k1, double k2, double d)
+{
+    constexpr double c5 = 5.0 / 12.0, c6 = 1.0 / 6.0, c3 = 1.0 / 3.0;
+    double Kf;
+    if constexpr (J == 0) Kf = c5 * k0 + c5 * k1 + c6 * k2;
+    else if constexpr (J == 1) Kf = c6 * k0 + c5 * k1 + c5 * k2;
+    else Kf = c5 * k0 + c6 * k1 + c5 * k2;
+    // (grad h) . n_j = ax * (d gx) + ay * (d gy); the gravity term (e_y . n_j) = ay * d
+    double gxd, gyd, ax, ay;
+    if constexpr (TYPE == 0) {
+        gxd = h1 - h0; gyd = h2 - h1;
+        ax = J == 0 ? c3 : -c6;
+        ay = J == 1 ? c3 : -c6;
+    } else {
+        gxd = h1 - h2; gyd = h2 - h0;
+        ax = J == 1 ? -c3 : c6;
+        ay = J == 2 ? -c3 : c6;
+    }
+    return -Kf * (gxd * ax + (gyd + d) * ay);
+}
+
+template <int TYPE, int LV>
+__device__ __forceinline__ void pair_struct(double& net, int e, int v0, int v1, int v2, const double* h,
+                                            const double* KA, const double* KB, const unsigned char* soil, double d)
+{
+    const double* K = soil[e] == TSR_SOIL_A ? KA : KB;
+    const double h0 = h[v0], h1 = h[v1], h2 = h[v2];
+    const double k0 = K[v0], k1 = K[v1], k2 = K[v2];
+    constexpr int JIN = LV == 0 ? 2 : LV - 1;
+    net += q_struct<TYPE, JIN>(h0, h1, h2, k0, k1, k2, d);
+    net -= q_struct<TYPE, LV>(h0, h1, h2, k0, k1, k2, d);
+    __builtin_amdgcn_sched_barrier(0);   // one pair at a time (register pressure)
+}
+
+__device__ __forceinline__ double struct_net(const DTmpl& T, int i, const double* h, const double* KA,
+                                             const double* KB, const unsigned char* soil)
+{
+    const int r = T.r, rm = r - 1;
+    const int iy = div_r(T, i), ix = i - iy * r;
+    const double d = T.dx;
+    const int iLD = i - r - 1, iD = i - r, iL = i - 1, iR = i + 1, iU = i + r, iRU = i + r + 1;
+    const int eLD = 2 * ((iy - 1) * rm + ix - 1), eRD = 2 * ((iy - 1) * rm + ix);
+    const int eLU = 2 * (iy * rm + ix - 1), eRU = 2 * (iy * rm + ix);
+    double net = 0.0;
+    pair_struct<0, 2>(net, eLD, iLD, iD, i, h, KA, KB, soil, d);
+    pair_struct<1, 1>(net, eLD + 1, iLD, i, iL, h, KA, KB, soil, d);
+    pair_struct<1, 2>(net, eRD + 1, iD, iR, i, h, KA, KB, soil, d);
+    pair_struct<0, 1>(net, eLU, iL, i, iU, h, KA, KB, soil, d);
+    pair_struct<0, 0>(net, eRU, i, iR, iRU, h, KA, KB, soil, d);
+    pair_struct<1, 0>(net, eRU + 1, i, iRU, iU, h, KA, KB, soil, d);
+    return net;
+}
+
 // Launch shape (tuned per problem by tune_micro): workgroup size bs (64..1024, a
 // multiple of 64, each thread owning nodes tid, tid + bs, ...; at most 8 of them) and a
 // register budget WPE (amdgpu_waves_per_eu: 1 = compiler default, 6, 8).  Results do
@@ -429,7 +496,12 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
     for (int i = tid; i < nn; i += bs) {
         {
             double gi = 0.0, net = 0.0;
-            if (!is_dirichlet<STRUCT>(T, i)) {
+            if constexpr (STRUCT) {
+                if (!is_dirichlet<STRUCT>(T, i)) {
+                    net = struct_net(T, i, h, KA, KB, soil);
+                    gi = net / stl[i];
+                }
+            } else if (!is_dirichlet<STRUCT>(T, i)) {
                 for_incident<STRUCT>(T, i, [&](int e, int lv) {
                     int v0, v1, v2;
                     if constexpr (STRUCT) {
@@ -621,6 +693,20 @@ static bool detect_structured(const micro_mesh_struct& m, const double* w, const
     if (st) {
         geo(0, sg[0]);
         geo(1, sg[1]);
+        // the analytic right-triangle geometry (k_micro's structured stencil): square
+        // cells of side d in both directions
+        const double d = m.nodes[1] - m.nodes[0];
+        const double an[2][21] = {
+            {-1 / d, 1 / d, 0, 0, -1 / d, 1 / d, d / 3, -d / 6, -d / 6, d / 3, -d / 6, -d / 6,
+             5. / 12, 5. / 12, 1. / 6, 1. / 6, 5. / 12, 5. / 12, 5. / 12, 1. / 6, 5. / 12},
+            {0, 1 / d, -1 / d, -1 / d, 0, 1 / d, d / 6, d / 6, -d / 3, d / 6, d / 6, -d / 3,
+             5. / 12, 5. / 12, 1. / 6, 1. / 6, 5. / 12, 5. / 12, 5. / 12, 1. / 6, 5. / 12}};
+        st = d > 0.0 && std::fabs((m.nodes[nn + r] - m.nodes[nn]) - d) <= 1e-12 * d;
+        for (int t = 0; t < 2 && st; ++t)
+            for (int k = 0; k < 21 && st; ++k) {
+                const double scale = k < 6 ? 1 / d : k < 12 ? d : 1.0;
+                st = std::fabs(sg[t][k] - an[t][k]) <= 1e-12 * scale;
+            }
         double g[22];
         for (int e = 0; st && e < ne; ++e) {
             geo(e, g);
@@ -658,7 +744,7 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
 {
     Arena A;
     struct TOff { size_t ptr, inc, el, sf, nrm, w, scv, vec, cen, bnd, soil, blist; int nn, ne; double ca; bool has_soil;
-                  int structured, r, nbnd; unsigned mul_r, mul_rm; double sg[2][22]; };
+                  int structured, r, nbnd; unsigned mul_r, mul_rm; double dx = 0.0; double sg[2][22]; };
     struct SOff { size_t keff, el, nodes, sf, nrm, midx, midy, cv, scv, bce, blen, bmid, bnrm, cen, ptr, inc, dir, alpha, hel, qe, bq; };
     std::vector<TOff> to(v.n_tmpl);
     std::vector<SOff> so(v.n_scen);
@@ -685,7 +771,7 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
             for (unsigned n = 0; exact && n < 4u * (unsigned)nn; ++n)
                 exact = (unsigned)(((unsigned long long)n * mr) >> 32) == n / (unsigned)r &&
                         (unsigned)(((unsigned long long)n * mrm) >> 32) == n / (unsigned)(r - 1);
-            if (exact) { o.structured = 1; o.r = r; o.mul_r = mr; o.mul_rm = mrm; }
+            if (exact) { o.structured = 1; o.r = r; o.mul_r = mr; o.mul_rm = mrm; o.dx = m.nodes[1] - m.nodes[0]; }
         }
         std::vector<int> blist;
         for (int i = 0; i < nn; ++i) if (m.boundary_nodes[i] == 1) blist.push_back(i);
@@ -788,6 +874,7 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
         T.r = o.r;
         T.mul_r = o.mul_r;
         T.mul_rm = o.mul_rm;
+        T.dx = o.dx;
         T.nbnd = o.nbnd;
         T.bnd_list = at<int>(base, o.blist);
         std::memcpy(T.sg, o.sg, sizeof(T.sg));

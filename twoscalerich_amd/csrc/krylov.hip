@@ -499,19 +499,20 @@ __global__ void k_start_finish(KrylovState K, int n_scen, const int* active)
 
 // Per-scenario state summary for the step controller: out[s*6 + 0..5] = mean, min, max of
 // the macro heads and of the matrix soil's effective saturation
-// Se(h) = (1 + (alpha |h|)^n)^(-m) (1 for h >= 0).  One workgroup per scenario; the
-// sums run in a fixed order (thread-strided, then the wavefront tree, then the four
-// wavefronts in order).
-__global__ __launch_bounds__(256) void k_macro_stats(const DScen* scen, const double* u, double* out)
+// Se(h) = (1 + (alpha |h|)^n)^(-m) (1 for h >= 0).  One 1024-thread workgroup per
+// scenario; the sums run in a fixed order (thread-strided, then the wavefront tree, then
+// the sixteen wavefronts in order).
+constexpr int kStatsBS = 1024;
+__global__ __launch_bounds__(kStatsBS) void k_macro_stats(const DScen* scen, const double* u, double* out)
 {
-    __shared__ double red[6][4];
+    __shared__ double red[6][kStatsBS / 64];
     const int s = blockIdx.x, tid = threadIdx.x;
     const DScen& S = scen[s];
     const double* h = u + S.u_off;
     double sh = 0.0, mnh = INFINITY, mxh = -INFINITY, ss = 0.0, mns = INFINITY, mxs = -INFINITY;
-    for (int j = tid; j < S.nn; j += 256) {
+    for (int j = tid; j < S.nn; j += kStatsBS) {
         const double hj = h[j];
-        const double se = hj < 0.0 ? exp(-S.A.m * log1p(pow(S.A.al * (-hj), S.A.n))) : 1.0;
+        const double se = hj < 0.0 ? exp(-S.A.m * log1p(exp(S.A.n * log(S.A.al * (-hj))))) : 1.0;
         sh += hj; mnh = fmin(mnh, hj); mxh = fmax(mxh, hj);
         ss += se; mns = fmin(mns, se); mxs = fmax(mxs, se);
     }
@@ -530,21 +531,85 @@ __global__ __launch_bounds__(256) void k_macro_stats(const DScen* scen, const do
     }
     __syncthreads();
     if (tid == 0) {
+        double a0 = 0.0, a3 = 0.0, a1 = INFINITY, a2 = -INFINITY, a4 = INFINITY, a5 = -INFINITY;
+        for (int w = 0; w < kStatsBS / 64; ++w) {
+            a0 += red[0][w]; a3 += red[3][w];
+            a1 = fmin(a1, red[1][w]); a2 = fmax(a2, red[2][w]);
+            a4 = fmin(a4, red[4][w]); a5 = fmax(a5, red[5][w]);
+        }
         double* o = out + (size_t)s * 6;
         const double n = (double)S.nn;
-        o[0] = (((red[0][0] + red[0][1]) + red[0][2]) + red[0][3]) / n;
-        o[1] = fmin(fmin(red[1][0], red[1][1]), fmin(red[1][2], red[1][3]));
-        o[2] = fmax(fmax(red[2][0], red[2][1]), fmax(red[2][2], red[2][3]));
-        o[3] = (((red[3][0] + red[3][1]) + red[3][2]) + red[3][3]) / n;
-        o[4] = fmin(fmin(red[4][0], red[4][1]), fmin(red[4][2], red[4][3]));
-        o[5] = fmax(fmax(red[5][0], red[5][1]), fmax(red[5][2], red[5][3]));
+        o[0] = a0 / n; o[1] = a1; o[2] = a2; o[3] = a3 / n; o[4] = a4; o[5] = a5;
+    }
+}
+
+// Readbacks in one launch.  The host syncs on the stream a few times per Arnoldi
+// window; instead of one copy per array, one kernel writes everything the host needs
+// into a mapped pinned host buffer (the host reads it after the sync).
+//
+// Window (scenario s, at out + s * stride): Hessenberg columns c0..jt-1, rows 0..col+1
+// each, packed in column order; cs[0..jt]; nwj[0..jt-1]; wrj[0..jt-1]; bad[s].
+__global__ void k_pack_window(KrylovState K, int c0, int jt, const int* bad, const int* active, double* out,
+                              int stride)
+{
+    const int s = blockIdx.x;
+    if (active && !active[s]) return;
+    double* o = out + (size_t)s * stride;
+    const double* H = K.H + (size_t)s * K.hstride;
+    int base = 0;
+    for (int col = c0; col < jt; ++col) {
+        for (int i = threadIdx.x; i <= col + 1; i += blockDim.x) o[base + i] = H[(size_t)col * K.ldh + i];
+        base += col + 2;
+    }
+    const double* cs = K.cs + (size_t)s * K.ld;
+    for (int i = threadIdx.x; i <= jt; i += blockDim.x) o[base + i] = cs[i];
+    base += jt + 1;
+    const double* nw = K.nwj + (size_t)s * K.ld;
+    const double* wr = K.wrj + (size_t)s * K.ld;
+    for (int i = threadIdx.x; i < jt; i += blockDim.x) {
+        o[base + i] = nw[i];
+        o[base + jt + i] = wr[i];
+    }
+    base += 2 * jt;
+    if (threadIdx.x == 0) o[base] = (double)bad[s];
+}
+
+// Per-scenario scalars (scenario s at out + s * 16): res[0..7], bad, eps, stats[0..5]
+// (eps and stats only when given).
+__global__ void k_pack_small(const double* res, const int* bad, const double* eps, const double* stats, int n_scen,
+                             double* out)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_scen) return;
+    double* o = out + (size_t)s * 16;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) o[k] = res[(size_t)s * 8 + k];
+    o[8] = (double)bad[s];
+    if (eps) o[9] = eps[s];
+    if (stats) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) o[10 + k] = stats[(size_t)s * 6 + k];
     }
 }
 
 // ---------------------------------------------------------------- launchers
+hipError_t launch_pack_window(const KrylovState& K, int c0, int jt, const int* bad, const int* active, int n_scen,
+                              double* out, int stride, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pack_window, dim3(n_scen), dim3(64), 0, st, K, c0, jt, bad, active, out, stride);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_small(const double* res, const int* bad, const double* eps, const double* stats, int n_scen,
+                             double* out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pack_small, dim3((n_scen + 63) / 64), dim3(64), 0, st, res, bad, eps, stats, n_scen, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_macro_stats(const DScen* scen, int n_scen, const double* u, double* out, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_macro_stats, dim3(n_scen), dim3(256), 0, st, scen, u, out);
+    hipLaunchKernelGGL(k_macro_stats, dim3(n_scen), dim3(kStatsBS), 0, st, scen, u, out);
     return hipGetLastError();
 }
 

@@ -117,14 +117,10 @@ struct tsr_solver {
     // timing of the last run
     double g_ms_acc = 0.0;
     int    g_calls = 0;
-    double* pin_h = nullptr;  // pinned host buffers for per-iteration readbacks
-    double* pin_res = nullptr;
-    int* pin_bad = nullptr;
     double* mstats = nullptr;   // device [S*6] macro head / saturation summary
-    double* pin_ms = nullptr;
-    double* pin_cs = nullptr;   // per-window readbacks: column scales, ||w||^2, WRMS per iteration
-    double* pin_nw = nullptr;
-    double* pin_wr = nullptr;
+    double* mb_h = nullptr;     // mapped pinned readback buffer (host view) ...
+    double* mb_d = nullptr;     // ... and its device address
+    int mb_stride = 0;          // doubles per scenario
     int path = TSR_PATH_PLAIN;
     double* linrec = nullptr;   // TSR_PATH_JVLIN: expansion point of the FD products (kLinComp x N)
     double t_dense = 0.0;     // host seconds in the small dense phi routines
@@ -224,23 +220,20 @@ int clear_bad(tsr_solver* s)
     return 0;
 }
 
-// Read back the per-scenario results (res, 8 slots), bad flags, and H column j of the basis.
-int read_iteration(tsr_solver* s, KrylovState& K, int j, std::vector<double>* Hcols)
+// Read back the per-scenario results (res, 8 slots) and bad flags, plus the FD step
+// and the macro statistics when eps / stats are given: one pack kernel into the mapped
+// buffer, one sync.
+int read_iteration(tsr_solver* s, KrylovState& K, double* eps_out = nullptr, bool stats = false)
 {
     const int fail_code = 2;
-    HIPCHK(hipMemcpyAsync(s->res_h.data(), K.res, sizeof(double) * 8 * s->S, hipMemcpyDeviceToHost, s->st));
-    HIPCHK(hipMemcpyAsync(s->bad_h.data(), s->bad, sizeof(int) * s->S, hipMemcpyDeviceToHost, s->st));
-    if (Hcols) {
-        std::vector<double> tmp((size_t)s->S * (j + 2));
-        HIPCHK(hipMemcpy2DAsync(tmp.data(), sizeof(double) * (j + 2), K.H + (size_t)j * K.ldh,
-                                sizeof(double) * K.hstride, sizeof(double) * (j + 2), s->S, hipMemcpyDeviceToHost,
-                                s->st));
-        HIPCHK(timed_sync(s));
-        for (int q = 0; q < s->S; ++q)
-            if (s->mask_h[q])
-                for (int i = 0; i <= j + 1; ++i) Hcols[q][(size_t)j * K.ldh + i] = tmp[(size_t)q * (j + 2) + i];
-    } else {
-        HIPCHK(timed_sync(s));
+    HIPCHK(launch_pack_small(K.res, s->bad, eps_out ? K.eps : nullptr, stats ? s->mstats : nullptr, s->S, s->mb_d, s->st));
+    HIPCHK(timed_sync(s));
+    for (int q = 0; q < s->S; ++q) {
+        const double* o = s->mb_h + (size_t)q * 16;
+        std::memcpy(&s->res_h[(size_t)q * 8], o, sizeof(double) * 8);
+        s->bad_h[q] = o[8] != 0.0;
+        if (eps_out) eps_out[q] = o[9];
+        if (stats) std::memcpy(s->sc[q].mstat, o + 10, sizeof(double) * 6);
     }
     return 0;
 }
@@ -265,22 +258,25 @@ double krylov_est_main(ScState& c, int m, double tau, int ldh, int ldphi)
 // enqueued back to back and the rule is then evaluated for every m of that window in
 // order; iterations beyond the first passing m are wasted work, not a different result.
 // Returns 0, or 2 on device error (an RHS failure sets the scenario's failed flag).
+// Parts of scenario q's window in the mapped buffer (layout of k_pack_window):
+// 0 packed H columns, 1 cs[0..jt], 2 nwj / wrj (jt each), 3 bad.
+const double* window_part(const tsr_solver* s, int q, int c0, int jt, int part)
+{
+    const double* o = s->mb_h + (size_t)q * s->mb_stride;
+    const long long nh = (long long)(jt - c0) * (c0 + jt + 3) / 2;   // sum over col of (col + 2)
+    if (part == 0) return o;
+    if (part == 1) return o + nh;
+    if (part == 2) return o + nh + jt + 1;
+    return o + nh + jt + 1 + 2 * jt;
+}
+
 int read_window(tsr_solver* s, KrylovState& K, int c0, int jt)
 {
     const int fail_code = 2;
-    const int S = s->S;
-    const int ncol = jt - c0;
-    HIPCHK(hipMemcpy2DAsync(s->pin_h, sizeof(double) * ncol * K.ldh, K.H + (size_t)c0 * K.ldh,
-                            sizeof(double) * K.hstride, sizeof(double) * ncol * K.ldh, S, hipMemcpyDeviceToHost, s->st));
-    HIPCHK(hipMemcpy2DAsync(s->pin_cs, sizeof(double) * (jt + 1), K.cs, sizeof(double) * K.ld,
-                            sizeof(double) * (jt + 1), S, hipMemcpyDeviceToHost, s->st));
-    HIPCHK(hipMemcpy2DAsync(s->pin_nw, sizeof(double) * jt, K.nwj, sizeof(double) * K.ld, sizeof(double) * jt, S,
-                            hipMemcpyDeviceToHost, s->st));
-    HIPCHK(hipMemcpy2DAsync(s->pin_wr, sizeof(double) * jt, K.wrj, sizeof(double) * K.ld, sizeof(double) * jt, S,
-                            hipMemcpyDeviceToHost, s->st));
-    HIPCHK(hipMemcpyAsync(s->pin_bad, s->bad, sizeof(int) * S, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(launch_pack_window(K, c0, jt, s->bad, s->mask, s->S, s->mb_d, s->mb_stride, s->st));
     HIPCHK(timed_sync(s));
-    std::memcpy(s->bad_h.data(), s->pin_bad, sizeof(int) * S);
+    for (int q = 0; q < s->S; ++q)
+        if (s->mask_h[q]) s->bad_h[q] = window_part(s, q, c0, jt, 3)[0] != 0.0;
     return 0;
 }
 
@@ -341,16 +337,17 @@ int arnoldi(tsr_solver* s, int which)
             if (s->bad_h[q]) { c.failed = true; s->mask_h[q] = 0; continue; }
             std::vector<double>& Hq = which == 0 ? c.H : c.HD;
             std::vector<double>& csq = which == 0 ? c.cs : c.csW;
-            const double* hw = s->pin_h + (size_t)q * (jt - c0) * K.ldh;
             // column col holds rows 0..col+1; the rows below are zero (Hessenberg)
+            const double* hw = window_part(s, q, c0, jt, 0);
             for (int col = c0; col < jt; ++col) {
                 double* hc = &Hq[(size_t)col * ldh];
-                for (int i = 0; i <= col + 1; ++i) hc[i] = hw[(size_t)(col - c0) * K.ldh + i];
+                for (int i = 0; i <= col + 1; ++i) hc[i] = hw[i];
+                hw += col + 2;
                 std::fill(hc + col + 2, hc + ldh, 0.0);
             }
-            std::memcpy(csq.data(), s->pin_cs + (size_t)q * (jt + 1), sizeof(double) * (jt + 1));
-            const double* nw = s->pin_nw + (size_t)q * jt;
-            const double* wr = s->pin_wr + (size_t)q * jt;
+            std::memcpy(csq.data(), window_part(s, q, c0, jt, 1), sizeof(double) * (jt + 1));
+            const double* nw = window_part(s, q, c0, jt, 2);
+            const double* wr = nw + jt;
             bool stop = false;
             for (int m = j0 + 1; m <= jt && !stop; ++m) {
                 const double hn = Hq[(size_t)(m - 1) * ldh + m];
@@ -423,11 +420,8 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
     if (launch_reduce(RED_START, s->part, s->ldp, 3, s->blk_first, S, s->scen_rows, s->KV, 0, s->mask, s->st) != hipSuccess)
         return -1;
     HIPCHK(launch_macro_stats(batch_of(s->P).scen, S, s->u, s->mstats, s->st));
-    HIPCHK(hipMemcpyAsync(s->pin_ms, s->mstats, sizeof(double) * 6 * S, hipMemcpyDeviceToHost, s->st));
     std::vector<double> eps_h(S);
-    HIPCHK(hipMemcpyAsync(eps_h.data(), s->KV.eps, sizeof(double) * S, hipMemcpyDeviceToHost, s->st));
-    if (read_iteration(s, s->KV, 0, nullptr)) return -1;
-    for (int q = 0; q < S; ++q) std::memcpy(s->sc[q].mstat, s->pin_ms + (size_t)q * 6, sizeof(double) * 6);
+    if (read_iteration(s, s->KV, eps_h.data(), true)) return -1;
     // proposal: the classical law's (kept in c.proposal), replaced by the learned
     // controller's when it is active and its output is usable
     if (features_and_proposal(s, act)) return -1;
@@ -518,7 +512,7 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
         HIPCHK(launch_lincomb(s->D, s->g2, s->Vd, s->nk, s->KV.coef, s->KV.ld, -1.0, s->M, s->nblk, s->part, s->ldp, 1,
                               s->u, o.AbsTol, o.RelTol, s->g0, s->mask, s->st));
         HIPCHK(launch_reduce(RED_SUM, s->part, s->ldp, 1, s->blk_first, S, s->scen_rows, s->KW, 0, s->mask, s->st));
-        if (read_iteration(s, s->KW, 0, nullptr)) return -1;
+        if (read_iteration(s, s->KW)) return -1;
         std::vector<int> okq(S, 0), dq(S, 0);
         for (int q = 0; q < S; ++q) {
             if (!pend[q]) continue;
@@ -577,7 +571,7 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
         HIPCHK(launch_lincomb(s->unew, s->u2, s->Wd, s->nk, s->KW.coef, s->KW.ld, 1.0, s->M, s->nblk, s->part, s->ldp, 2,
                               s->u, o.AbsTol, o.RelTol, nullptr, s->mask, s->st));
         HIPCHK(launch_reduce(RED_SUM, s->part, s->ldp, 1, s->blk_first, S, s->scen_rows, s->KW, 0, s->mask, s->st));
-        if (read_iteration(s, s->KW, 0, nullptr)) return -1;
+        if (read_iteration(s, s->KW)) return -1;
         std::vector<int> acc(S, 0);
         int nacc = 0;
         for (int q = 0; q < S; ++q) {
@@ -840,13 +834,12 @@ tsr_solver* create(const tsr_batch_view* view, const exprem_options_struct* opt)
         c.cs.assign(s->mcap + 2, 0.0);
         c.csW.assign(s->mcap + 2, 0.0);
     }
-    if (hipHostMalloc((void**)&s->pin_h, sizeof(double) * (size_t)S * s->mcap * (s->mcap + 1)) != hipSuccess ||
-        hipHostMalloc((void**)&s->pin_cs, sizeof(double) * (size_t)S * (s->mcap + 2)) != hipSuccess ||
-        hipHostMalloc((void**)&s->pin_nw, sizeof(double) * (size_t)S * (s->mcap + 2)) != hipSuccess ||
-        hipHostMalloc((void**)&s->pin_wr, sizeof(double) * (size_t)S * (s->mcap + 2)) != hipSuccess ||
-        hipHostMalloc((void**)&s->pin_res, sizeof(double) * (size_t)S * 8) != hipSuccess ||
-        hipHostMalloc((void**)&s->pin_bad, sizeof(int) * (size_t)S) != hipSuccess ||
-        hipHostMalloc((void**)&s->pin_ms, sizeof(double) * (size_t)S * 6) != hipSuccess ||
+    // mapped readback buffer: a full window (mcap columns) per scenario, at least 16 doubles
+    s->mb_stride = std::max(16, s->mcap * (s->mcap + 3) / 2 + (s->mcap + 1) + 2 * s->mcap + 1);
+    s->mb_stride = (s->mb_stride + 7) & ~7;
+    if (hipHostMalloc((void**)&s->mb_h, sizeof(double) * (size_t)S * s->mb_stride,
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&s->mb_d, s->mb_h, 0) != hipSuccess ||
         dalloc(&s->mstats, (size_t)S * 6)) {
         set_error("hipHostMalloc failed");
         return nullptr;
@@ -887,13 +880,7 @@ void tsr_solver_free(tsr_solver* s)
     }
     hipFree(s->KV.eps); hipFree(s->KV.ieps);
     hipFree(s->mask); hipFree(s->nk);
-    if (s->pin_h) hipHostFree(s->pin_h);
-    if (s->pin_res) hipHostFree(s->pin_res);
-    if (s->pin_bad) hipHostFree(s->pin_bad);
-    if (s->pin_ms) hipHostFree(s->pin_ms);
-    if (s->pin_cs) hipHostFree(s->pin_cs);
-    if (s->pin_nw) hipHostFree(s->pin_nw);
-    if (s->pin_wr) hipHostFree(s->pin_wr);
+    if (s->mb_h) hipHostFree(s->mb_h);
     if (s->linrec) hipFree(s->linrec);
     if (s->mstats) hipFree(s->mstats);
     if (s->ev0) hipEventDestroy(s->ev0);
