@@ -94,6 +94,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--chunk", type=int, default=1000, help="EXPREM steps between progress lines")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only time the G kernels in their three modes on a perturbed initial state (for profilers)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -115,6 +117,12 @@ def main():
     fast = T.Solver(p, opt)
     fast.set_path(F.PATH_BATCHED)
     CT.attach(fast)
+    if args.kernels_only:
+        rng = np.random.default_rng(1)
+        fast.set_state(u0 + 0.5 * rng.standard_normal(p.N), 0.0)
+        print(json.dumps({"gfunc": gfunc_modes(T, fast, args.kernel_reps)}), flush=True)
+        fast.close()
+        return
     plain = T.Solver(p, opt)     # plain path, classical controller (the defaults)
 
     for _ in range(args.warmup):
@@ -155,14 +163,7 @@ def main():
                                         "source": os.path.relpath(PLAIN_FULL_FILE, ROOT)}
 
     # G kernels per mode and the dominant kernel's roofline (HIP events on the solver stream)
-    du = L.tsr_solver_dev_u(fast.h)
-    g = L.tsr_solver_dev_scratch(fast.h, 1)
-    gf = {}
-    for mode, name in ((0, "full"), (1, "record"), (2, "linearised_fd")):
-        ms3 = (C.c_double * 3)()
-        L.tsr_solver_time_gfunc_mode(fast.h, g, du, mode, 3, ms3)
-        L.tsr_solver_time_gfunc_mode(fast.h, g, du, mode, args.kernel_reps, ms3)
-        gf[name] = {"macro_elem_ms": round(ms3[0], 4), "micro_ms": round(ms3[1], 4), "macro_node_ms": round(ms3[2], 4)}
+    gf = gfunc_modes(T, fast, args.kernel_reps)
     v = p.view.contents
     M = v.scen[0].ud.macro_mesh.no_nodes
     n_micro = p.N - M
@@ -235,6 +236,22 @@ def main():
     print(json.dumps(line), flush=True)
     fast.close()
     plain.close()
+
+
+def gfunc_modes(T, s, reps):
+    """Per-kernel device times of G about the solver's state in its three k_micro modes
+    (0 full, 1 recording the expansion point, 2 linearised FD product), HIP events on the
+    solver stream; the first launches tune the micro kernel's launch shape per mode."""
+    L = T.lib()
+    du = L.tsr_solver_dev_u(s.h)
+    g = L.tsr_solver_dev_scratch(s.h, 1)
+    gf = {}
+    for mode, name in ((0, "full"), (1, "record"), (2, "linearised_fd")):
+        ms3 = (C.c_double * 3)()
+        L.tsr_solver_time_gfunc_mode(s.h, g, du, mode, 3, ms3)
+        L.tsr_solver_time_gfunc_mode(s.h, g, du, mode, reps, ms3)
+        gf[name] = {"macro_elem_ms": round(ms3[0], 4), "micro_ms": round(ms3[1], 4), "macro_node_ms": round(ms3[2], 4)}
+    return gf
 
 
 def cpu_baseline(T, p, opt, tend, budget_s):

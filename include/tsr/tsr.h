@@ -126,6 +126,7 @@ typedef struct tsr_step_record {
     double tau_first;    /* step of the first attempt (after the Krylov-budget halvings) */
     double err_first;    /* WRMS error estimate of the first attempt (inf if its RHS failed) */
     float  feat[16];     /* the controller's inputs at the step start (order above) */
+    double wall_ms;      /* host wall time of the step (to the end of its last sync) */
     int    m;            /* Krylov dimension of phi_1 */
     int    mD;           /* Krylov dimension of the error estimate */
     int    rejects;      /* rejected attempts before acceptance */

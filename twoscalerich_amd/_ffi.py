@@ -75,7 +75,7 @@ class tsr_step_record(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("t", "tau", "tau_proposed", "err", "beta", "hnorm", "resid",
                                           "head_mean", "head_min", "head_max", "sat_mean", "sat_min", "sat_max",
                                           "tau_first", "err_first")] + \
-               [("feat", C.c_float * 16)] + \
+               [("feat", C.c_float * 16), ("wall_ms", C.c_double)] + \
                [(n, C.c_int) for n in ("m", "mD", "rejects", "fevals", "controller", "m_cap")]
 
 

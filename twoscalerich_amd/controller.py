@@ -51,7 +51,7 @@ O_B3 = O_W3 + N_OUT * N_H
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "controller_weights.npz")
 
 REC_FIELDS = ("t", "tau", "tau_proposed", "err", "tau_first", "err_first", "m", "mD", "rejects", "fevals",
-              "controller", "m_cap")
+              "controller", "m_cap", "wall_ms")
 
 
 def _fp(a):

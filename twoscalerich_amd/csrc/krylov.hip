@@ -174,19 +174,27 @@ __global__ __launch_bounds__(kVecBS) void k_pass_dca(const double* const* cols, 
         rj[r] = ok[r] ? Rj[idx[r]] : 0.0;
     }
     int k = 0;
-    for (; k + 2 <= j; k += 2) {
-        const double* c0 = cols[k];
-        const double* c1 = cols[k + 1];
-        double s0 = 0.0, s1 = 0.0, p0 = 0.0, p1 = 0.0;
+    for (; k + 4 <= j; k += 4) {   // four columns at a time: 16 loads in flight per thread
+        double sx[4] = {0.0, 0.0, 0.0, 0.0}, px[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int r = 0; r < kVecR; ++r)
             if (ok[r]) {
-                const double a0 = c0[idx[r]], a1 = c1[idx[r]];
-                s0 += a0 * wr[r]; s1 += a1 * wr[r];
-                p0 += a0 * rj[r]; p1 += a1 * rj[r];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const double a = cols[k + c][idx[r]];
+                    sx[c] += a * wr[r];
+                    px[c] += a * rj[r];
+                }
             }
-        s0 = wave_sum_v(s0); s1 = wave_sum_v(s1); p0 = wave_sum_v(p0); p1 = wave_sum_v(p1);
-        if (lane == 0) { lds[wv][k] = s0; lds[wv][k + 1] = s1; lds[wv][nk + k] = p0; lds[wv][nk + k + 1] = p1; }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            sx[c] = wave_sum_v(sx[c]);
+            px[c] = wave_sum_v(px[c]);
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) { lds[wv][k + c] = sx[c]; lds[wv][nk + k + c] = px[c]; }
+        }
     }
     for (; k < j; ++k) {
         const double* c0 = cols[k];
@@ -222,11 +230,17 @@ __global__ __launch_bounds__(kVecBS) void k_pass_dca(const double* const* cols, 
 // res[1] and nwj[j] receive ||w||^2 (before projection).
 __global__ void k_dc_finish(KrylovState K, int j, int n_scen, const int* active)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n_scen || (active && !active[s])) return;
+    // one workgroup per scenario: the inputs are staged in LDS by all threads, then
+    // thread 0 does the (order-dependent, O(j)) arithmetic
+    __shared__ double sd[2 * kMaxCols + 2], scs[kMaxCols + 1];
+    const int s = blockIdx.x;
+    if (active && !active[s]) return;
     const int nk = j + 1;
-    const double* d = K.dots + (size_t)s * K.ld2;
-    double* cs = K.cs + (size_t)s * K.ld;
+    for (int i = threadIdx.x; i <= 2 * nk; i += blockDim.x) sd[i] = K.dots[(size_t)s * K.ld2 + i];
+    for (int i = threadIdx.x; i <= j; i += blockDim.x) scs[i] = K.cs[(size_t)s * K.ld + i];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double* d = sd;
     double* coef = K.coef + (size_t)s * K.ld;
     double* coefr = K.coefr + (size_t)s * K.ld;
     double* H = K.H + (size_t)s * K.hstride;
@@ -234,39 +248,40 @@ __global__ void k_dc_finish(KrylovState K, int j, int n_scen, const int* active)
     K.res[(size_t)s * 8 + 1] = d[2 * nk];
     K.nwj[(size_t)s * K.ld + j] = d[2 * nk];
     if (j == 0) {
-        const double h0 = cs[0] * d[0];
+        const double h0 = scs[0] * d[0];
         hj_col[0] = h0;
-        coef[0] = cs[0] * h0;
+        coef[0] = scs[0] * h0;
         return;
     }
-    const double csj = cs[j];
+    __shared__ double sa[kMaxCols], sh[kMaxCols];   // a_k, h_k
+    const double csj = scs[j];
     double suma2 = 0.0;
     for (int k = 0; k < j; ++k) {
-        const double a = cs[k] * csj * d[nk + k];
-        coefr[k] = a;   // a_k for now
+        const double a = scs[k] * csj * d[nk + k];
+        sa[k] = a;
         suma2 += a * a;
     }
     const double g2 = csj * csj * d[2 * nk - 1] - suma2;
     const double gam = (csj > 0.0 && g2 > 0.0) ? sqrt(g2) : 1.0;
     double* hp = H + (size_t)(j - 1) * K.ldh;
     const double hprev = hp[j];
-    for (int k = 0; k < j; ++k) hp[k] += coefr[k] * hprev;
+    for (int k = 0; k < j; ++k) hp[k] += sa[k] * hprev;
     hp[j] = hprev * gam;
     double ah = 0.0;
     for (int k = 0; k < j; ++k) {
-        const double hk = cs[k] * d[k];
+        const double hk = scs[k] * d[k];
+        sh[k] = hk;
         hj_col[k] = hk;
-        ah += coefr[k] * hk;
+        ah += sa[k] * hk;
     }
     const double hjj = (csj * d[j] - ah) / gam;
     hj_col[j] = hjj;
     for (int k = 0; k < j; ++k) {
-        const double a = coefr[k];
-        coef[k] = (hj_col[k] - hjj * a / gam) * cs[k];
-        coefr[k] = csj > 0.0 ? a * cs[k] / csj : 0.0;
+        coef[k] = (sh[k] - hjj * sa[k] / gam) * scs[k];
+        coefr[k] = csj > 0.0 ? sa[k] * scs[k] / csj : 0.0;
     }
     coef[j] = hjj * csj / gam;
-    cs[j] = csj / gam;
+    K.cs[(size_t)s * K.ld + j] = csj / gam;
 }
 
 // w -= sum_{k<=j} coef_k R_k;  R_j -= sum_{k<j} coefr_k R_k (both from the old R_j);
@@ -297,7 +312,27 @@ __global__ __launch_bounds__(kVecBS) void k_pass_dcb(const double* const* cols, 
         wr[r] = ok[r] ? w[idx[r]] - cj * rj[r] : 0.0;
         rn[r] = rj[r];
     }
-    for (int k = 0; k < j; ++k) {
+    int k = 0;
+    for (; k + 4 <= j; k += 4) {   // four columns at a time: 16 loads in flight per thread
+        const double* c0 = cols[k];
+        const double* c1 = cols[k + 1];
+        const double* c2p = cols[k + 2];
+        const double* c3 = cols[k + 3];
+#pragma unroll
+        for (int r = 0; r < kVecR; ++r)
+            if (ok[r]) {
+                const double a0 = c0[idx[r]], a1 = c1[idx[r]], a2 = c2p[idx[r]], a3 = c3[idx[r]];
+                wr[r] -= a0 * cf[k];
+                rn[r] -= a0 * cr[k];
+                wr[r] -= a1 * cf[k + 1];
+                rn[r] -= a1 * cr[k + 1];
+                wr[r] -= a2 * cf[k + 2];
+                rn[r] -= a2 * cr[k + 2];
+                wr[r] -= a3 * cf[k + 3];
+                rn[r] -= a3 * cr[k + 3];
+            }
+    }
+    for (; k < j; ++k) {
         const double c = cf[k], c2 = cr[k];
         const double* col = cols[k];
 #pragma unroll
@@ -658,7 +693,7 @@ hipError_t launch_dcgs2(int nk, const double* const* cols, double* w, const doub
     hipLaunchKernelGGL(k_pass_dca, dim3(nblk), dim3(kVecBS), 0, st, cols, nk, w, M, part, ldp, active);
     hipLaunchKernelGGL(k_reduce, dim3(n_scen, 2 * nk + 1), dim3(256), 0, st, (int)RED_COLS, part, ldp, 2 * nk + 1,
                        blk_first, scen_rows, K, j, active);
-    hipLaunchKernelGGL(k_dc_finish, dim3((n_scen + 63) / 64), dim3(64), 0, st, K, j, n_scen, active);
+    hipLaunchKernelGGL(k_dc_finish, dim3(n_scen), dim3(64), 0, st, K, j, n_scen, active);
     hipLaunchKernelGGL(k_pass_dcb, dim3(nblk), dim3(kVecBS), 0, st, cols, nk, w, u, K.coef, K.coefr, K.ld, M, part, ldp,
                        active, at, rt);
     hipLaunchKernelGGL(k_reduce, dim3(n_scen, 2), dim3(256), 0, st, (int)RED_NORM, part, ldp, 2, blk_first, scen_rows, K,

@@ -1038,8 +1038,11 @@ int tsr_solver_run(tsr_solver* s, double tend, int max_steps, tsr_run_stats* out
         bool done = true;
         for (auto& c : s->sc) done = done && !(c.t < tend);
         if (done) break;
+        const auto ts = std::chrono::steady_clock::now();
         rc = solver_step(s, tend, rec.data());
         if (rc) break;
+        const double wms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts).count();
+        for (int q = 0; q < s->S; ++q) rec[q].wall_ms = wms;
         for (int q = 0; q < s->S; ++q) { rej += rec[q].rejects; fe += rec[q].fevals; }
         if (records && steps * s->S < max_records)
             for (int q = 0; q < s->S && steps * s->S + q < max_records; ++q) records[steps * s->S + q] = rec[q];
