@@ -220,12 +220,17 @@ int   tsr_mlp_train(void* mlp, const float* x, const float* t, int n, int epochs
  *   TSR_PATH_DEFERRED  Arnoldi iterations enqueued back to back up to the predicted
  *                      Krylov dimension; the stopping rule is then evaluated for every
  *                      dimension of the window in order (same first passing m)
+ *   TSR_PATH_MONOTONE  with DEFERRED: evaluate the stopping rule at the window's last
+ *                      dimension first and scan back only when it passes (the rule is
+ *                      then satisfied from the chosen m to the window's end; equals the
+ *                      first passing m when the rule's estimate decreases with m)
  *   TSR_PATH_JVLIN     FD products with the micro nodes' capacities and conductivities
  *                      expanded to first order about the step's u (the FD quotient then
  *                      differs from the full one by O(eps) terms, as the FD quotient
  *                      itself differs from J v)
  * Returns 0, 1 on unknown bits, 2 on allocation failure. */
-enum { TSR_PATH_PLAIN = 0, TSR_PATH_DCGS2 = 1, TSR_PATH_DEFERRED = 2, TSR_PATH_JVLIN = 4, TSR_PATH_BATCHED = 7 };
+enum { TSR_PATH_PLAIN = 0, TSR_PATH_DCGS2 = 1, TSR_PATH_DEFERRED = 2, TSR_PATH_JVLIN = 4, TSR_PATH_MONOTONE = 8,
+       TSR_PATH_BATCHED = 15 };
 int tsr_solver_set_path(tsr_solver* s, int path);
 /* jv = (G(u + eps v) - G(u)) / eps about the solver's state u, with the full RHS
  * (lin = 0) or the TSR_PATH_JVLIN expansion (lin = 1); v, jv host arrays of N. */

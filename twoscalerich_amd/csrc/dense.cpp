@@ -19,8 +19,12 @@ namespace tsr {
 
 namespace {
 
+// The hot loops are built twice: for x86-64-v3 (AVX2, FMA; picked at load time where
+// the CPU has it) and for baseline x86-64.
+#define TSR_HOST_CLONES __attribute__((target_clones("arch=x86-64-v3", "default")))
+
 // C <- alpha * A * B + beta * C, n x n column-major.
-void mm(int n, double alpha, const double* A, const double* B, double beta, double* C, double* tmp)
+TSR_HOST_CLONES void mm(int n, double alpha, const double* A, const double* B, double beta, double* C, double* tmp)
 {
     for (int j = 0; j < n; ++j) {
         double* t = tmp + (size_t)j * n;
@@ -49,9 +53,10 @@ double inf_norm(int n, const double* A)
 }
 
 // Gaussian elimination with partial pivoting; X overwrites B (n x nrhs).
-bool solve(int n, double* A, double* B, int nrhs)
+TSR_HOST_CLONES bool solve(int n, double* A, double* B, int nrhs)
 {
-    std::vector<int> p(n);
+    int p[256];   // n <= 2 * kMaxKrylovCap + 3 (augmented matrices)
+    if (n > 256) return false;
     for (int k = 0; k < n; ++k) {
         int piv = k;
         double big = std::fabs(A[k + (size_t)k * n]);

@@ -172,10 +172,11 @@ hipError_t launch_dcgs2(int nk, const double* const* cols, double* w, const doub
                        const int* active, double at, double rt, hipStream_t st);
 hipError_t launch_masked_copy(double* dst, const double* src, const BlkMap& M, int nblk, const int* mask,
                               hipStream_t st);
-// Readbacks into a mapped pinned host buffer (layouts in krylov.hip)
-hipError_t launch_pack_window(const KrylovState& K, int c0, int jt, const int* bad, const int* active, int n_scen,
+// Readbacks into a mapped pinned host buffer (layouts in krylov.hip); both clear the
+// bad flags they read
+hipError_t launch_pack_window(const KrylovState& K, int c0, int jt, int* bad, const int* active, int n_scen,
                               double* out, int stride, hipStream_t st);
-hipError_t launch_pack_small(const double* res, const int* bad, const double* eps, const double* stats, int n_scen,
+hipError_t launch_pack_small(const double* res, int* bad, const double* eps, const double* stats, int n_scen,
                              double* out, hipStream_t st);
 // out[s*6 + 0..5]: mean, min, max macro head; mean, min, max matrix-soil saturation
 hipError_t launch_macro_stats(const DScen* scen, int n_scen, const double* u, double* out, hipStream_t st);

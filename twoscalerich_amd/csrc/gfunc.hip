@@ -428,14 +428,19 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
     }
     for (int i = tid; i < nn; i += bs) {
         if (is_dirichlet<STRUCT>(T, i)) continue;
-        const double hi = ld_state(a, ve, off + i);
-        h[i] = hi;
         if constexpr (MODE == 2) {
-            const double* L = a.lin + off + i;
-            const double d = hi - L[0];
-            stl[i] = L[NN] + L[2 * NN] * d;
-            KA[i] = L[3 * NN] + L[4 * NN] * d;
-            KB[i] = L[5 * NN] + L[6 * NN] * d;
+            // interior nodes: the perturbation itself is the offset from the expansion
+            // point (the recorded head is u), so component 0 is not read here
+            const long long gi = off + i;
+            const double dv = a.v ? ve * a.v[gi] : 0.0;
+            const double hi = a.u[gi] + dv;
+            h[i] = hi;
+            const double* L = a.lin + gi;
+            stl[i] = L[NN] + L[2 * NN] * dv;
+            KA[i] = L[3 * NN] + L[4 * NN] * dv;
+            KB[i] = L[5 * NN] + L[6 * NN] * dv;
+        } else {
+            h[i] = ld_state(a, ve, off + i);
         }
     }
     for (int e = tid; e < ne; e += bs) soil[e] = (unsigned char)elem_soil(B, T, cell, e);

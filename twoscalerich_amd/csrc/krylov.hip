@@ -584,7 +584,7 @@ __global__ __launch_bounds__(kStatsBS) void k_macro_stats(const DScen* scen, con
 //
 // Window (scenario s, at out + s * stride): Hessenberg columns c0..jt-1, rows 0..col+1
 // each, packed in column order; cs[0..jt]; nwj[0..jt-1]; wrj[0..jt-1]; bad[s].
-__global__ void k_pack_window(KrylovState K, int c0, int jt, const int* bad, const int* active, double* out,
+__global__ void k_pack_window(KrylovState K, int c0, int jt, int* bad, const int* active, double* out,
                               int stride)
 {
     const int s = blockIdx.x;
@@ -606,12 +606,15 @@ __global__ void k_pack_window(KrylovState K, int c0, int jt, const int* bad, con
         o[base + jt + i] = wr[i];
     }
     base += 2 * jt;
-    if (threadIdx.x == 0) o[base] = (double)bad[s];
+    if (threadIdx.x == 0) {
+        o[base] = (double)bad[s];
+        bad[s] = 0;   // read and cleared: the next window starts clean
+    }
 }
 
 // Per-scenario scalars (scenario s at out + s * 16): res[0..7], bad, eps, stats[0..5]
 // (eps and stats only when given).
-__global__ void k_pack_small(const double* res, const int* bad, const double* eps, const double* stats, int n_scen,
+__global__ void k_pack_small(const double* res, int* bad, const double* eps, const double* stats, int n_scen,
                              double* out)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -620,6 +623,7 @@ __global__ void k_pack_small(const double* res, const int* bad, const double* ep
 #pragma unroll
     for (int k = 0; k < 8; ++k) o[k] = res[(size_t)s * 8 + k];
     o[8] = (double)bad[s];
+    bad[s] = 0;
     if (eps) o[9] = eps[s];
     if (stats) {
 #pragma unroll
@@ -628,14 +632,14 @@ __global__ void k_pack_small(const double* res, const int* bad, const double* ep
 }
 
 // ---------------------------------------------------------------- launchers
-hipError_t launch_pack_window(const KrylovState& K, int c0, int jt, const int* bad, const int* active, int n_scen,
+hipError_t launch_pack_window(const KrylovState& K, int c0, int jt, int* bad, const int* active, int n_scen,
                               double* out, int stride, hipStream_t st)
 {
     hipLaunchKernelGGL(k_pack_window, dim3(n_scen), dim3(64), 0, st, K, c0, jt, bad, active, out, stride);
     return hipGetLastError();
 }
 
-hipError_t launch_pack_small(const double* res, const int* bad, const double* eps, const double* stats, int n_scen,
+hipError_t launch_pack_small(const double* res, int* bad, const double* eps, const double* stats, int n_scen,
                              double* out, hipStream_t st)
 {
     hipLaunchKernelGGL(k_pack_small, dim3((n_scen + 63) / 64), dim3(64), 0, st, res, bad, eps, stats, n_scen, out);

@@ -103,6 +103,7 @@ struct tsr_solver {
     int* nk = nullptr;        // per-scenario column counts for k_lincomb
     int* bad = nullptr;       // alias of DBatch::bad
     std::vector<int> mask_h, nk_h, bad_h;
+    std::vector<int> mask_dev_h;   // what the device mask holds (uploads only on change)
     std::vector<double> coef_h, res_h, veps_h, cs0_h;
     std::vector<ScState> sc;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -209,7 +210,9 @@ int G_at(tsr_solver* s, const double* x, double* out, const int* active)
 int upload_mask(tsr_solver* s)
 {
     const int fail_code = 2;
+    if (s->mask_dev_h == s->mask_h) return 0;   // the device copy is current
     HIPCHK(hipMemcpyAsync(s->mask, s->mask_h.data(), sizeof(int) * s->S, hipMemcpyHostToDevice, s->st));
+    s->mask_dev_h = s->mask_h;
     return 0;
 }
 
@@ -290,6 +293,7 @@ int arnoldi(tsr_solver* s, int which)
     const int ldh = s->mcap + 1;
     const bool dc = (s->path & TSR_PATH_DCGS2) != 0;
     const bool defer = (s->path & TSR_PATH_DEFERRED) != 0;
+    const bool monotone = (s->path & TSR_PATH_MONOTONE) != 0;
     const double at = s->opt.AbsTol, rt = s->opt.RelTol;
     std::vector<int> cap(S, 0), next(S, 0);
     for (int q = 0; q < S; ++q) {
@@ -307,7 +311,8 @@ int arnoldi(tsr_solver* s, int which)
         for (int q = 0; q < S; ++q)
             if (s->mask_h[q]) jt = std::min(jt, next[q]);
         if (jt == INT_MAX) break;
-        if (upload_mask(s) || clear_bad(s)) return 2;
+        // (the bad flags were cleared by the previous readback)
+        if (upload_mask(s)) return 2;
         const int j0 = j;
         for (; j < jt; ++j) {
             if (G_call(s, B[j], K.veps, s->g0, K.ieps, B[j + 1], s->mask)) return 2;
@@ -348,33 +353,62 @@ int arnoldi(tsr_solver* s, int which)
             std::memcpy(csq.data(), window_part(s, q, c0, jt, 1), sizeof(double) * (jt + 1));
             const double* nw = window_part(s, q, c0, jt, 2);
             const double* wr = nw + jt;
-            bool stop = false;
-            for (int m = j0 + 1; m <= jt && !stop; ++m) {
+            // the stopping rule at dimension m (state for m set in c as a side effect)
+            auto passes = [&](int m, bool& bd) {
                 const double hn = Hq[(size_t)(m - 1) * ldh + m];
                 const double wnorm0 = std::sqrt(nw[m - 1]);
                 const double wn = wr[m - 1] / hn;
-                const bool bd = !(hn > 1e-12 * wnorm0) || hn == 0.0;
+                bd = !(hn > 1e-12 * wnorm0) || hn == 0.0;
                 if (which == 0) {
                     c.m = m; c.hn = hn; c.wnext = wn;
-                    if (bd) { c.bd = true; c.conv = true; stop = true; break; }
-                    if (m >= s->opt.MinKrylov && krylov_est_main(c, m, c.tau, ldh, s->mcap) <= kKrylovTol) {
-                        c.conv = true;
+                    if (bd) return true;
+                    return m >= s->opt.MinKrylov && krylov_est_main(c, m, c.tau, ldh, s->mcap) <= kKrylovTol;
+                }
+                c.mD = m; c.hnD = hn; c.wnextD = wn;
+                if (bd) return true;
+                if (m < s->opt.MinKrylov) return false;
+                std::vector<double> ph(3 * (size_t)m);
+                int prc;
+                {
+                    Tic tic(dense_acc());
+                    prc = phik_e1(ph.data(), c.HD.data(), ldh, m, c.tau, 3);
+                }
+                if (prc) return false;
+                return 2.0 * c.tau * c.betaD * (c.tau * hn * std::fabs(ph[2 * m + m - 1])) * wn <= kKrylovTol;
+            };
+            auto finish = [&](int m, bool bd) {   // stop at m
+                bool b2;
+                passes(m, b2);
+                if (which == 0) { c.bd = bd; c.conv = true; } else { c.bdD = bd; }
+            };
+            auto c_m_of = [](const ScState& cc, int w) { return w == 0 ? cc.m : cc.mD; };
+            auto finish_flags = [](ScState& cc, int w) { if (w == 0) { cc.bd = false; cc.conv = true; } else { cc.bdD = false; } };
+            bool stop = false;
+            if (monotone && jt - j0 > 1) {
+                // breakdowns first (cheap), then the rule at the window's end; only if it
+                // passes, scan back for the smallest m from which it passes up to jt
+                int mb = 0;
+                for (int m = j0 + 1; m <= jt && !mb; ++m) {
+                    const double hn = Hq[(size_t)(m - 1) * ldh + m];
+                    if (!(hn > 1e-12 * std::sqrt(nw[m - 1])) || hn == 0.0) mb = m;
+                }
+                bool bd;
+                if (mb) {
+                    finish(mb, true);
+                    stop = true;
+                } else if (passes(jt, bd)) {
+                    int m0 = jt;
+                    while (m0 - 1 > j0 && passes(m0 - 1, bd)) --m0;
+                    if (m0 != c_m_of(c, which)) finish(m0, false);   // the scan left c at m0 - 1
+                    else finish_flags(c, which);
+                    stop = true;
+                }   // else: c holds the window's end (not converged)
+            } else {
+                for (int m = j0 + 1; m <= jt && !stop; ++m) {
+                    bool bd;
+                    if (passes(m, bd)) {
+                        if (which == 0) { c.bd = bd; c.conv = true; } else { c.bdD = bd; }
                         stop = true;
-                    }
-                } else {
-                    c.mD = m; c.hnD = hn; c.wnextD = wn;
-                    if (bd) { c.bdD = true; stop = true; break; }
-                    if (m >= s->opt.MinKrylov) {
-                        std::vector<double> ph(3 * (size_t)m);
-                        int prc;
-                        {
-                            Tic tic(dense_acc());
-                            prc = phik_e1(ph.data(), c.HD.data(), ldh, m, c.tau, 3);
-                        }
-                        if (!prc) {
-                            const double est = 2.0 * c.tau * c.betaD * (c.tau * hn * std::fabs(ph[2 * m + m - 1])) * wn;
-                            if (est <= kKrylovTol) stop = true;
-                        }
                     }
                 }
             }
@@ -490,8 +524,7 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
         HIPCHK(hipMemcpyAsync(s->nk, s->nk_h.data(), sizeof(int) * S, hipMemcpyHostToDevice, s->st));
         HIPCHK(launch_lincomb(s->u2, s->u, s->Vd, s->nk, s->KV.coef, s->KV.ld, 1.0, s->M, s->nblk, s->part, s->ldp, 0,
                               s->u, o.AbsTol, o.RelTol, nullptr, s->mask, s->st));
-        if (clear_bad(s)) return -1;
-        if (G_at(s, s->u2, s->g2, s->mask)) return -1;
+        if (G_at(s, s->u2, s->g2, s->mask)) return -1;   // bad flags cleared by the last readback
         // D = g2 - g0 - V_{m+1} Hbar y
         for (int q = 0; q < S; ++q) {
             ScState& c = s->sc[q];
