@@ -313,6 +313,47 @@ __device__ __forceinline__ double struct_net(const DTmpl& T, int i, const double
     return net;
 }
 
+// Net inflow of node i of a generic template: its incident (element, local vertex)
+// pairs from the CSR in ascending element order, the two sub-face fluxes each pair
+// needs, net += q_in; net -= q_out.
+__device__ __forceinline__ double generic_net(const DTmpl& T, int i, const double* h, const double* KA,
+                                              const double* KB, const unsigned char* soil)
+{
+    const int ne = T.ne;
+    double net = 0.0;
+    for (int qq = T.ptr[i]; qq < T.ptr[i + 1]; ++qq) {
+        const int e = T.inc[qq] >> 2, lv = T.inc[qq] & 3;
+        const int v0 = T.el[e], v1 = T.el[ne + e], v2 = T.el[2 * ne + e];
+        const double h0 = h[v0], h1 = h[v1], h2 = h[v2];
+        const double gx = T.sf[e] * h0 + T.sf[ne + e] * h1 + T.sf[2 * ne + e] * h2;
+        const double gy = T.sf[3 * ne + e] * h0 + T.sf[4 * ne + e] * h1 + T.sf[5 * ne + e] * h2;
+        const double* Ks = (soil[e] == TSR_SOIL_A) ? KA : KB;
+        const double k0 = Ks[v0], k1 = Ks[v1], k2 = Ks[v2];
+        const int jin = lv == 0 ? 2 : lv - 1;   // (lv + 2) % 3: the sub-face entering v_lv
+        double qj[2];
+#pragma unroll
+        for (int t2 = 0; t2 < 2; ++t2) {
+            const int j = t2 == 0 ? jin : lv;
+            const double Kf = T.w[(j * 3) * ne + e] * k0 + T.w[(j * 3 + 1) * ne + e] * k1 + T.w[(j * 3 + 2) * ne + e] * k2;
+            qj[t2] = -Kf * (gx * T.nrm[(2 * j) * ne + e] + (gy + 1.0) * T.nrm[(2 * j + 1) * ne + e]);
+        }
+        net += qj[0];
+        net -= qj[1];
+    }
+    return net;
+}
+
+// Linearised FD products keep each thread's nodes (tid + k bs, k < kLinNPT) in
+// registers between the load phase and the gather (g0 is prefetched with the state).
+constexpr int kLinNPT = 4;
+
+__device__ __forceinline__ void store_fd(const DBatch& B, const GArgs& a, int s, long long i, double g, double g0v)
+{
+    g = (g - g0v) * a.ieps[s];
+    if (!isfinite(g)) atomicOr(B.bad + s, 1);
+    a.out[i] = g;
+}
+
 // Launch shape (tuned per problem by tune_micro): workgroup size bs (64..1024, a
 // multiple of 64, each thread owning nodes tid, tid + bs, ...; at most 8 of them) and a
 // register budget WPE (amdgpu_waves_per_eu: 1 = compiler default, 6, 8).  Results do
@@ -426,12 +467,18 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
             KB[i] = L[5 * NN] + L[6 * NN] * d;
         }
     }
-    for (int i = tid; i < nn; i += bs) {
-        if (is_dirichlet<STRUCT>(T, i)) continue;
-        if constexpr (MODE == 2) {
+    double g0r[kLinNPT];   // MODE 2: g0 of this thread's nodes
+    if constexpr (MODE == 2) {
+#pragma unroll
+        for (int k = 0; k < kLinNPT; ++k) {
+            const int i = tid + k * bs;
+            g0r[k] = 0.0;
+            if (i >= nn) continue;
+            const long long gi = off + i;
+            g0r[k] = a.g0[gi];
+            if (is_dirichlet<STRUCT>(T, i)) continue;
             // interior nodes: the perturbation itself is the offset from the expansion
             // point (the recorded head is u), so component 0 is not read here
-            const long long gi = off + i;
             const double dv = a.v ? ve * a.v[gi] : 0.0;
             const double hi = a.u[gi] + dv;
             h[i] = hi;
@@ -439,9 +486,10 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
             stl[i] = L[NN] + L[2 * NN] * dv;
             KA[i] = L[3 * NN] + L[4 * NN] * dv;
             KB[i] = L[5 * NN] + L[6 * NN] * dv;
-        } else {
-            h[i] = ld_state(a, ve, off + i);
         }
+    } else {
+        for (int i = tid; i < nn; i += bs)
+            if (!is_dirichlet<STRUCT>(T, i)) h[i] = ld_state(a, ve, off + i);
     }
     for (int e = tid; e < ne; e += bs) soil[e] = (unsigned char)elem_soil(B, T, cell, e);
     __syncthreads();
@@ -498,63 +546,28 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
     // fluxes it needs, q_j = -K_j (grad h . n_j + n_j,y) (q_j leaves v_j, enters
     // v_{j+1}): net = sum q_{(lv+2)%3} - q_{lv}.  Each sub-face flux is the same
     // expression whichever node evaluates it, so the result does not depend on who does.
-    for (int i = tid; i < nn; i += bs) {
-        {
-            double gi = 0.0, net = 0.0;
-            if constexpr (STRUCT) {
-                if (!is_dirichlet<STRUCT>(T, i)) {
-                    net = struct_net(T, i, h, KA, KB, soil);
-                    gi = net / stl[i];
-                }
-            } else if (!is_dirichlet<STRUCT>(T, i)) {
-                for_incident<STRUCT>(T, i, [&](int e, int lv) {
-                    int v0, v1, v2;
-                    if constexpr (STRUCT) {
-                        const int sq = e >> 1, r = T.r;
-                        const int row = div_rm(T, sq);
-                        const int a0 = row * r + (sq - row * (r - 1));
-                        v0 = a0;
-                        v1 = (e & 1) ? a0 + r + 1 : a0 + 1;
-                        v2 = (e & 1) ? a0 + r : a0 + r + 1;
-                    } else {
-                        v0 = T.el[e]; v1 = T.el[ne + e]; v2 = T.el[2 * ne + e];
-                    }
-                    const double h0 = h[v0], h1 = h[v1], h2 = h[v2];
-                    double gx, gy;
-                    if constexpr (STRUCT) {
-                        const double* g = s_geo[e & 1];
-                        gx = g[0] * h0 + g[1] * h1 + g[2] * h2;
-                        gy = g[3] * h0 + g[4] * h1 + g[5] * h2;
-                    } else {
-                        gx = T.sf[e] * h0 + T.sf[ne + e] * h1 + T.sf[2 * ne + e] * h2;
-                        gy = T.sf[3 * ne + e] * h0 + T.sf[4 * ne + e] * h1 + T.sf[5 * ne + e] * h2;
-                    }
-                    const double* Ks = (soil[e] == TSR_SOIL_A) ? KA : KB;
-                    const double k0 = Ks[v0], k1 = Ks[v1], k2 = Ks[v2];
-                    const int jin = lv == 0 ? 2 : lv - 1;   // (lv + 2) % 3: the sub-face entering v_lv
-                    double qj[2];
+    auto node_net = [&](int i) -> double {
+        if constexpr (STRUCT) return struct_net(T, i, h, KA, KB, soil);
+        else return generic_net(T, i, h, KA, KB, soil);
+    };
+    if constexpr (MODE == 2) {
 #pragma unroll
-                    for (int t2 = 0; t2 < 2; ++t2) {
-                        const int j = t2 == 0 ? jin : lv;
-                        double Kf, nx, ny;
-                        if constexpr (STRUCT) {
-                            const double* g = s_geo[e & 1];
-                            Kf = g[12 + j * 3] * k0 + g[12 + j * 3 + 1] * k1 + g[12 + j * 3 + 2] * k2;
-                            nx = g[6 + 2 * j];
-                            ny = g[6 + 2 * j + 1];
-                        } else {
-                            Kf = T.w[(j * 3) * ne + e] * k0 + T.w[(j * 3 + 1) * ne + e] * k1 + T.w[(j * 3 + 2) * ne + e] * k2;
-                            nx = T.nrm[(2 * j) * ne + e];
-                            ny = T.nrm[(2 * j + 1) * ne + e];
-                        }
-                        qj[t2] = -Kf * (gx * nx + (gy + 1.0) * ny);
-                    }
-                    net += qj[0];
-                    net -= qj[1];
-                    // one incident pair at a time: keeps the structured visitor's six
-                    // straight-line visits from hoisting every LDS load (register spills)
-                    __builtin_amdgcn_sched_barrier(0);
-                });
+        for (int k = 0; k < kLinNPT; ++k) {
+            const int i = tid + k * bs;
+            if (i >= nn) continue;
+            double gi = 0.0, net = 0.0;
+            if (!is_dirichlet<STRUCT>(T, i)) {
+                net = node_net(i);
+                gi = net / stl[i];
+            }
+            stl[i] = net;   // only this thread reads or writes entry i in this phase
+            store_fd(B, a, s, off + i, gi, g0r[k]);
+        }
+    } else {
+        for (int i = tid; i < nn; i += bs) {
+            double gi = 0.0, net = 0.0;
+            if (!is_dirichlet<STRUCT>(T, i)) {
+                net = node_net(i);
                 gi = net / stl[i];
             }
             stl[i] = net;   // only this thread reads or writes entry i in this phase
@@ -998,8 +1011,8 @@ static hipError_t tune_micro(DevProblem* p, const GArgs& a, hipStream_t st)
     const int mode = micro_mode(a);
     std::vector<int> cand_bs;
     for (int bs : {64, 128, 192, 256, 384, 512, 768, 1024}) {
-        // at most 8 strided nodes per thread
-        if (bs <= kMicroMaxBS && bs * 8 >= B.nn_max) cand_bs.push_back(bs);
+        // at most 8 strided nodes per thread (kLinNPT in the linearised mode)
+        if (bs <= kMicroMaxBS && bs * (mode == 2 ? kLinNPT : 8) >= B.nn_max) cand_bs.push_back(bs);
     }
     if (cand_bs.empty()) cand_bs.push_back(kMicroMaxBS);
     hipEvent_t e0, e1;
