@@ -125,6 +125,8 @@ size_t      micro_lds_bytes(const DevProblem* p);
 hipError_t  launch_gfunc(const DevProblem* p, const GArgs& a, hipStream_t st);
 // one of the three G kernels: 0 macro elements, 1 micro cells, 2 macro nodes
 hipError_t  launch_gfunc_part(const DevProblem* p, const GArgs& a, int part, hipStream_t st);
+// the linearised mode needs micro templates of at most 1024 nodes
+bool        lin_mode_ok(const DevProblem* p);
 // tuned micro-cell launch shape (bs = 0 before the first G)
 void        micro_shape(const DevProblem* p, int* bs, int* wpe);
 

@@ -382,8 +382,10 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
     double* h = smem;
     double* KA = h + B.nn_max;
     double* KB = KA + B.nn_max;
-    double* stl = KB + B.nn_max;      // node storage capacity; then the node's net inflow
-    unsigned char* soil = reinterpret_cast<unsigned char*>(stl + B.nn_max);
+    // MODE 0/1: node storage capacity, then the node's net inflow.  MODE 2 keeps both in
+    // registers (each thread owns its nodes from load to gather) and has no such array.
+    double* stl = KB + B.nn_max;
+    unsigned char* soil = reinterpret_cast<unsigned char*>(MODE == 2 ? stl : stl + B.nn_max);
     const double ve = a.v ? a.veps[s] : 0.0;
     const long long off = B.cell_off[cell];
     const int host = cell - S.first_cell;
@@ -467,12 +469,13 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
             KB[i] = L[5 * NN] + L[6 * NN] * d;
         }
     }
-    double g0r[kLinNPT];   // MODE 2: g0 of this thread's nodes
+    double g0r[kLinNPT], str[kLinNPT];   // MODE 2: g0 and storage of this thread's nodes
     if constexpr (MODE == 2) {
 #pragma unroll
         for (int k = 0; k < kLinNPT; ++k) {
             const int i = tid + k * bs;
             g0r[k] = 0.0;
+            str[k] = 1.0;
             if (i >= nn) continue;
             const long long gi = off + i;
             g0r[k] = a.g0[gi];
@@ -483,7 +486,7 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
             const double hi = a.u[gi] + dv;
             h[i] = hi;
             const double* L = a.lin + gi;
-            stl[i] = L[NN] + L[2 * NN] * dv;
+            str[k] = L[NN] + L[2 * NN] * dv;
             KA[i] = L[3 * NN] + L[4 * NN] * dv;
             KB[i] = L[5 * NN] + L[6 * NN] * dv;
         }
@@ -551,6 +554,11 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
         else return generic_net(T, i, h, KA, KB, soil);
     };
     if constexpr (MODE == 2) {
+        // the exchange slots (node i -> slot i % 256) this thread owns: with bs dividing
+        // 256, node tid + k bs falls in slot tid + (k % (256 / bs)) bs, and each slot's
+        // nodes are added in increasing order, as in the LDS version below
+        double qs[kLinNPT] = {0.0, 0.0, 0.0, 0.0};
+        const int nslot = 256 / bs;
 #pragma unroll
         for (int k = 0; k < kLinNPT; ++k) {
             const int i = tid + k * bs;
@@ -558,11 +566,24 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
             double gi = 0.0, net = 0.0;
             if (!is_dirichlet<STRUCT>(T, i)) {
                 net = node_net(i);
-                gi = net / stl[i];
+                gi = net / str[k];
             }
-            stl[i] = net;   // only this thread reads or writes entry i in this phase
+#pragma unroll
+            for (int j2 = 0; j2 < kLinNPT; ++j2)
+                if (j2 == k % nslot) qs[j2] += net;
             store_fd(B, a, s, off + i, gi, g0r[k]);
         }
+#pragma unroll
+        for (int j2 = 0; j2 < kLinNPT; ++j2)
+            if (j2 < nslot) s_part[tid + j2 * bs] = qs[j2];
+        __syncthreads();
+        if (tid < 64) {
+            double r4[4];
+#pragma unroll
+            for (int w = 0; w < 4; ++w) r4[w] = wave_sum(s_part[w * 64 + tid]);
+            if (tid == 0) B.Q[cell] = (r4[0] + r4[1]) + (r4[2] + r4[3]);
+        }
+        return;
     } else {
         for (int i = tid; i < nn; i += bs) {
             double gi = 0.0, net = 0.0;
@@ -637,7 +658,7 @@ struct DevProblem {
     void* buf = nullptr;              // one HBM allocation
     size_t bytes = 0;
     DBatch B{};
-    size_t lds = 0;
+    size_t lds = 0, lds2 = 0;           // dynamic LDS of k_micro (modes 0/1, mode 2)
     bool all_structured = false;      // every template is a uniform grid: analytic k_micro
     int micro_bs[3] = {0, 0, 0};      // tuned launch shape per k_micro mode (0 = not tuned yet)
     int micro_wpe[3] = {1, 1, 1};
@@ -944,6 +965,8 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
     for (int t = 0; t < v.n_tmpl; ++t) P->all_structured = P->all_structured && to[t].structured;
     P->lds = (size_t)4 * nn_max * sizeof(double) + (size_t)ne_max;
     P->lds = (P->lds + 15) & ~size_t(15);
+    P->lds2 = (size_t)3 * nn_max * sizeof(double) + (size_t)ne_max;   // linearised mode: no storage array
+    P->lds2 = (P->lds2 + 15) & ~size_t(15);
     if (P->lds > 64 * 1024) {
         bool ok = true;
         for (const void* f : micro_kernels()) ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds) == hipSuccess;
@@ -998,7 +1021,7 @@ static void launch_micro(const DevProblem* p, const GArgs& a, int bs, int wpe, h
 {
     const DBatch& B = p->B;
     MicroFn f = micro_fn(p->all_structured, micro_mode(a), wpe);
-    hipLaunchKernelGGL(f, dim3(B.n_cells), dim3(bs), p->lds, st, B, a);
+    hipLaunchKernelGGL(f, dim3(B.n_cells), dim3(bs), micro_mode(a) == 2 ? p->lds2 : p->lds, st, B, a);
 }
 
 // Autotuner: on the first G of a problem (per k_micro mode), time every admissible
@@ -1012,7 +1035,9 @@ static hipError_t tune_micro(DevProblem* p, const GArgs& a, hipStream_t st)
     std::vector<int> cand_bs;
     for (int bs : {64, 128, 192, 256, 384, 512, 768, 1024}) {
         // at most 8 strided nodes per thread (kLinNPT in the linearised mode)
-        if (bs <= kMicroMaxBS && bs * (mode == 2 ? kLinNPT : 8) >= B.nn_max) cand_bs.push_back(bs);
+        // (the linearised mode: bs divides 256, at most kLinNPT nodes per thread)
+        const bool ok2 = mode != 2 || (256 % bs == 0 && bs * kLinNPT >= B.nn_max);
+        if (bs <= kMicroMaxBS && bs * 8 >= B.nn_max && ok2) cand_bs.push_back(bs);
     }
     if (cand_bs.empty()) cand_bs.push_back(kMicroMaxBS);
     hipEvent_t e0, e1;
@@ -1051,6 +1076,7 @@ hipError_t launch_gfunc_part(const DevProblem* p, const GArgs& a, int part, hipS
         hipLaunchKernelGGL(k_macro_elem, ge, dim3(64), 0, st, B, a);
     } else if (part == 1) {
         const int mode = micro_mode(a);
+        if (mode == 2 && !lin_mode_ok(p)) return hipErrorInvalidValue;
         if (p->micro_bs[mode] == 0) return tune_micro(const_cast<DevProblem*>(p), a, st);
         launch_micro(p, a, p->micro_bs[mode], p->micro_wpe[mode], st);
     } else {
@@ -1059,6 +1085,8 @@ hipError_t launch_gfunc_part(const DevProblem* p, const GArgs& a, int part, hipS
     }
     return hipGetLastError();
 }
+
+bool lin_mode_ok(const DevProblem* p) { return p->B.nn_max <= 256 * kLinNPT; }
 
 void micro_shape(const DevProblem* p, int* bs, int* wpe)
 {

@@ -1122,6 +1122,10 @@ int tsr_solver_jv(tsr_solver* s, const double* v_host, double eps, int lin, doub
 int tsr_solver_set_path(tsr_solver* s, int path)
 {
     if (path & ~TSR_PATH_BATCHED) return 1;
+    if ((path & TSR_PATH_JVLIN) && !lin_mode_ok(s->P)) {
+        set_error("linearised FD products need micro templates of at most 1024 nodes");
+        return 1;
+    }
     if ((path & TSR_PATH_JVLIN) && !s->linrec) {
         if (hipMalloc((void**)&s->linrec, sizeof(double) * kLinComp * (size_t)s->N) != hipSuccess) {
             s->linrec = nullptr;
