@@ -778,8 +778,19 @@ int features_and_proposal(tsr_solver* s, const std::vector<int>& act)
     return 0;
 }
 
+// The step driver syncs on its stream a few times per Arnoldi window and reacts to the
+// result at once: ask for spin-waiting syncs (lower wake-up latency than yielding).
+// This only takes effect before the process's first HIP call creates the device
+// context; a caller that created it already keeps its own choice.
+void prefer_spin_sync()
+{
+    static std::once_flag once;
+    std::call_once(once, [] { (void)hipSetDeviceFlags(hipDeviceScheduleSpin); (void)hipGetLastError(); });
+}
+
 tsr_solver* create(const tsr_batch_view* view, const exprem_options_struct* opt)
 {
+    prefer_spin_sync();
     std::string err;
     std::unique_ptr<tsr_solver> s(new tsr_solver());
     s->opt = *opt;
