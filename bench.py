@@ -24,8 +24,9 @@ Extra fields on the JSON line:
                 plain path's full-horizon time from profiles/ for reference.
   gfunc         G evaluation times per kernel for the three micro-kernel modes.
   roofline      the dominant kernel of the timed run (k_micro in its linearised mode, the
-                FD products): algorithmic bytes of one launch (88 B per interior micro
-                node: u, v, 7 expansion components, g0, out; 56 B per Dirichlet node) over
+                FD products): algorithmic bytes of one launch (80 B per interior micro
+                node: u, v, 6 expansion components, g0, out; 56 B per Dirichlet node:
+                recorded head, 4 conductivity components, g0, out) over
                 its average duration from HIP events on the solver stream; peak 8 TB/s;
                 traffic = FETCH+WRITE bytes per launch from rocprofv3 counters (profiles/),
                 scaled by the copy-kernel calibration, or null.
@@ -120,7 +121,10 @@ def main():
     if args.kernels_only:
         rng = np.random.default_rng(1)
         fast.set_state(u0 + 0.5 * rng.standard_normal(p.N), 0.0)
-        print(json.dumps({"gfunc": gfunc_modes(T, fast, args.kernel_reps)}), flush=True)
+        copy_ms = C.c_double()
+        L.tsr_solver_time_copy(fast.h, args.kernel_reps, C.byref(copy_ms))
+        print(json.dumps({"gfunc": gfunc_modes(T, fast, args.kernel_reps),
+                          "copy_16N_ms": round(copy_ms.value, 4), "copy_bytes": 16 * p.N}), flush=True)
         fast.close()
         return
     plain = T.Solver(p, opt)     # plain path, classical controller (the defaults)
@@ -171,7 +175,7 @@ def main():
     n_cells = v.n_cells
     n_dir = n_cells * (4 * (int(round(r ** 0.5)) - 1))
     n_int = n_micro - n_dir
-    lin_bytes = 88.0 * n_int + 56.0 * n_dir + p.distinct_bytes()
+    lin_bytes = 80.0 * n_int + 56.0 * n_dir + p.distinct_bytes()
     lin_ms = gf["linearised_fd"]["micro_ms"]
     achieved = lin_bytes / (lin_ms * 1e-3) / 1e9
     full_bytes = 16.0 * p.N + p.distinct_bytes()

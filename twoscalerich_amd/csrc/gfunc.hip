@@ -347,6 +347,42 @@ __device__ __forceinline__ double generic_net(const DTmpl& T, int i, const doubl
 // registers between the load phase and the gather (g0 is prefetched with the state).
 constexpr int kLinNPT = 4;
 
+// What a linearised FD product reads of one micro node: g0 at every node; at interior
+// nodes the state, the direction and recorded components 1..6 (storage, its derivative,
+// K_A, K_A', K_B, K_B').  Loading into this register block first and applying later lets
+// a thread issue every load of its nodes before it waits on any of them.
+struct LinNode {
+    double g0, u, v, c[6];
+};
+
+template <bool STRUCT>
+__device__ __forceinline__ void lin_load(const DTmpl& T, const GArgs& a, size_t NN, long long off, int nn, int i,
+                                         LinNode& x)
+{
+    if (i >= nn) return;
+    const long long gi = off + i;
+    x.g0 = a.g0[gi];
+    if (is_dirichlet<STRUCT>(T, i)) return;
+    x.u = a.u[gi];
+    x.v = a.v ? a.v[gi] : 0.0;
+    const double* L = a.lin + gi;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) x.c[c] = L[(c + 1) * NN];
+}
+
+// Recorded head and the four conductivity components at a Dirichlet node.
+struct LinDir {
+    double hb, c[4];
+};
+
+__device__ __forceinline__ void lin_load_dir(const GArgs& a, size_t NN, long long gi, LinDir& x)
+{
+    const double* L = a.lin + gi;
+    x.hb = L[0];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) x.c[c] = L[(c + 3) * NN];
+}
+
 __device__ __forceinline__ void store_fd(const DBatch& B, const GArgs& a, int s, long long i, double g, double g0v)
 {
     g = (g - g0v) * a.ieps[s];
@@ -391,6 +427,18 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
     const int host = cell - S.first_cell;
     const int model = S.model;
     const bool node_host = (S.host == TSR_HOST_NODE);
+    const size_t NN = (size_t)B.N;
+    const int nbnd = STRUCT ? 4 * (T.r - 1) : T.nbnd;
+
+    // MODE 2: this thread's first two nodes and its first Dirichlet node are loaded
+    // before the set-up phase below, so those loads are in flight during it
+    LinNode xn[2];
+    LinDir xd;
+    if constexpr (MODE == 2) {
+        lin_load<STRUCT>(T, a, NN, off, nn, tid, xn[0]);
+        lin_load<STRUCT>(T, a, NN, off, nn, tid + bs, xn[1]);
+        if (tid < nbnd) lin_load_dir(a, NN, off + dirichlet_node<STRUCT>(T, tid), xd);
+    }
 
     // host interpolation data
     if (!node_host) {
@@ -422,11 +470,35 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
     }
     __syncthreads();
 
+    // MODE 2: interior heads, storage capacities (registers) and conductivities (LDS)
+    // from the recorded expansion, then the Dirichlet nodes below with their macro-field
+    // heads; everything is ready after one barrier.
+    double g0r[kLinNPT], str[kLinNPT];   // MODE 2: g0 and storage of this thread's nodes
+    if constexpr (MODE == 2) {
+#pragma unroll
+        for (int k = 0; k < kLinNPT; ++k) {
+            const int i = tid + k * bs;
+            g0r[k] = 0.0;
+            str[k] = 1.0;
+            if (i >= nn) continue;
+            // nodes 0 and 1 were loaded before the set-up phase; nodes 2 and 3 (templates
+            // with more than two nodes per thread) are loaded here
+            LinNode& x = xn[k & 1];
+            if (k >= 2) lin_load<STRUCT>(T, a, NN, off, nn, i, x);
+            g0r[k] = x.g0;
+            if (is_dirichlet<STRUCT>(T, i)) continue;
+            // interior nodes: the perturbation itself is the offset from the expansion
+            // point (the recorded head is u), so component 0 is not read here
+            const double dv = a.v ? ve * x.v : 0.0;
+            const double hi = x.u + dv;
+            h[i] = hi;
+            str[k] = x.c[0] + x.c[1] * dv;
+            KA[i] = x.c[2] + x.c[3] * dv;
+            KB[i] = x.c[4] + x.c[5] * dv;
+        }
+    }
+
     // heads: Dirichlet nodes from the macro field (first nbnd threads), the rest from u.
-    // In MODE 2 each thread also applies the recorded expansion to the nodes it loads,
-    // so capacities and conductivities are ready with the heads (one phase, one barrier).
-    const size_t NN = (size_t)B.N;
-    const int nbnd = STRUCT ? 4 * (T.r - 1) : T.nbnd;
     for (int b = tid; b < nbnd; b += bs) {
         const int i = dirichlet_node<STRUCT>(T, b);
         double hi;
@@ -463,34 +535,13 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
         }
         h[i] = hi;
         if constexpr (MODE == 2) {
-            const double* L = a.lin + off + i;
-            const double d = hi - L[0];
-            KA[i] = L[3 * NN] + L[4 * NN] * d;
-            KB[i] = L[5 * NN] + L[6 * NN] * d;
+            if (b != tid) lin_load_dir(a, NN, off + i, xd);
+            const double d = hi - xd.hb;
+            KA[i] = xd.c[0] + xd.c[1] * d;
+            KB[i] = xd.c[2] + xd.c[3] * d;
         }
     }
-    double g0r[kLinNPT], str[kLinNPT];   // MODE 2: g0 and storage of this thread's nodes
-    if constexpr (MODE == 2) {
-#pragma unroll
-        for (int k = 0; k < kLinNPT; ++k) {
-            const int i = tid + k * bs;
-            g0r[k] = 0.0;
-            str[k] = 1.0;
-            if (i >= nn) continue;
-            const long long gi = off + i;
-            g0r[k] = a.g0[gi];
-            if (is_dirichlet<STRUCT>(T, i)) continue;
-            // interior nodes: the perturbation itself is the offset from the expansion
-            // point (the recorded head is u), so component 0 is not read here
-            const double dv = a.v ? ve * a.v[gi] : 0.0;
-            const double hi = a.u[gi] + dv;
-            h[i] = hi;
-            const double* L = a.lin + gi;
-            str[k] = L[NN] + L[2 * NN] * dv;
-            KA[i] = L[3 * NN] + L[4 * NN] * dv;
-            KB[i] = L[5 * NN] + L[6 * NN] * dv;
-        }
-    } else {
+    if constexpr (MODE != 2) {
         for (int i = tid; i < nn; i += bs)
             if (!is_dirichlet<STRUCT>(T, i)) h[i] = ld_state(a, ve, off + i);
     }
