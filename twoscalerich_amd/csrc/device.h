@@ -90,6 +90,10 @@ struct DBatch {
     const double* discs;
     double* Q;                        // [n_cells]
     int* bad;                         // [n_scen] set when G produced a non-finite value
+    // node-hosted cells: for micro Dirichlet node i of cell c, entry c * nn_max + i is the
+    // index (into the host node's incident macro elements) of the element whose P1
+    // interpolant gives the node's head; fixed by the geometry, filled once at upload
+    unsigned char* dir_elem;
     double Ss;
     long long N;
 };

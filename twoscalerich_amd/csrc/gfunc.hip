@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -171,6 +172,55 @@ __global__ __launch_bounds__(256) void k_macro_elem(DBatch B, GArgs a)
             S.bq[e * 2 * nv + 2 * p + hk] = f;
         }
     }
+}
+
+// Barycentric coordinates of (px, py) in an incident macro element whose shape-function
+// coefficients are m[0..8] (gradient x, gradient y, constant for each vertex).
+__device__ __forceinline__ void macro_bary(const double* m, double px, double py, double l[3])
+{
+#pragma unroll
+    for (int v = 0; v < 3; ++v) l[v] = m[v] * px + m[3 + v] * py + m[6 + v];
+}
+
+// Micro Dirichlet node i of a node-hosted cell: its position, clamped to the macro
+// domain's bounding box.
+__device__ __forceinline__ void dir_point(const DScen& S, const DTmpl& T, double hx, double hy, int i, double& px,
+                                          double& py)
+{
+    px = fmin(fmax(hx + T.vec[i], S.box[0]), S.box[1]);
+    py = fmin(fmax(hy + T.vec[T.nn + i], S.box[2]), S.box[3]);
+}
+
+// Upload-time choice of the macro element that interpolates each micro Dirichlet node
+// of a node-hosted cell: among the host node's incident elements, the first with the
+// largest smallest barycentric coordinate (ties within 1e-12 keep the earlier one), so
+// a point on a shared edge takes the lower-numbered element.  One thread per
+// (cell, node); the same expressions as the interpolation in k_micro.
+__global__ __launch_bounds__(256) void k_dir_elem(DBatch B)
+{
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)B.n_cells * B.nn_max) return;
+    const int cell = (int)(id / B.nn_max), i = (int)(id % B.nn_max);
+    const DScen& S = B.scen[B.cell_scen[cell]];
+    const DTmpl& T = B.tmpl[B.cell_tmpl[cell]];
+    int best = 0;
+    if (i < T.nn && T.bnd[i] == 1 && S.host == TSR_HOST_NODE && S.model != 1) {
+        const int host = cell - S.first_cell;
+        const int q0 = S.ptr[host], ninc = S.ptr[host + 1] - q0;
+        double px, py;
+        dir_point(S, T, S.nodes[host], S.nodes[S.nn + host], i, px, py);
+        double bmin = -INFINITY;
+        for (int k = 0; k < ninc; ++k) {
+            const int e = S.inc[q0 + k] >> 2;
+            double m[9], l[3];
+#pragma unroll
+            for (int j = 0; j < 9; ++j) m[j] = S.sf[j * S.ne + e];
+            macro_bary(m, px, py, l);
+            const double mn = fmin(fmin(l[0], l[1]), l[2]);
+            if (k == 0 || mn > bmin + 1e-12) { bmin = mn; best = k; }
+        }
+    }
+    B.dir_elem[id] = (unsigned char)best;
 }
 
 // ---------------------------------------------------------------- micro cells
@@ -404,7 +454,6 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
     extern __shared__ __align__(16) double smem[];
     __shared__ double s_bc[6];
     __shared__ double s_mel[kMaxNodeInc][12];
-    __shared__ int s_ninc;
     __shared__ double s_part[256];
     __shared__ double s_geo[2][22];
     const int cell = blockIdx.x;
@@ -434,6 +483,8 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
     // before the set-up phase below, so those loads are in flight during it
     LinNode xn[2];
     LinDir xd;
+    int best = 0;   // node-hosted cells: macro element of this thread's first Dirichlet node
+    if (node_host && model != 1 && tid < nbnd) best = B.dir_elem[(size_t)cell * B.nn_max + dirichlet_node<STRUCT>(T, tid)];
     if constexpr (MODE == 2) {
         lin_load<STRUCT>(T, a, NN, off, nn, tid, xn[0]);
         lin_load<STRUCT>(T, a, NN, off, nn, tid + bs, xn[1]);
@@ -452,7 +503,6 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
     } else {
         const int q0 = S.ptr[host], ninc = S.ptr[host + 1] - q0;
         if (tid == 0) {
-            s_ninc = ninc;
             s_bc[3] = macro_head(S, a, ve, host);
             s_bc[4] = S.nodes[host];
             s_bc[5] = S.nodes[S.nn + host];
@@ -506,19 +556,12 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
             if (model == 1) {
                 hi = s_bc[3];
             } else {
-                const double px = fmin(fmax(s_bc[4] + T.vec[i], S.box[0]), S.box[1]);
-                const double py = fmin(fmax(s_bc[5] + T.vec[nn + i], S.box[2]), S.box[3]);
-                int best = -1;
-                double bmin = -INFINITY, lam[3] = {0, 0, 0};
-                for (int k = 0; k < s_ninc; ++k) {
-                    double l[3], mn = INFINITY;
-#pragma unroll
-                    for (int v = 0; v < 3; ++v) {
-                        l[v] = s_mel[k][v] * px + s_mel[k][3 + v] * py + s_mel[k][6 + v];
-                        mn = fmin(mn, l[v]);
-                    }
-                    if (best < 0 || mn > bmin + 1e-12) { bmin = mn; best = k; lam[0] = l[0]; lam[1] = l[1]; lam[2] = l[2]; }
-                }
+                // P1 interpolation in the incident macro element chosen at upload
+                // (k_dir_elem): weights max(lambda, 0), normalised
+                double px, py, lam[3];
+                dir_point(S, T, s_bc[4], s_bc[5], i, px, py);
+                if (b != tid) best = B.dir_elem[(size_t)cell * B.nn_max + i];
+                macro_bary(s_mel[best], px, py, lam);
                 double ws = 0.0, hs = 0.0;
 #pragma unroll
                 for (int v = 0; v < 3; ++v) {
@@ -898,7 +941,10 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
             std::vector<int> ptr, inc;
             csr(nn, ne, nv, M.elements, ptr, inc);
             for (int j = 0; j < nn; ++j)
-                if (sc.host_kind == TSR_HOST_NODE && ptr[j + 1] - ptr[j] > kMaxNodeInc) { err = "macro node with too many elements"; return nullptr; }
+                if (sc.host_kind == TSR_HOST_NODE && (ptr[j + 1] - ptr[j] > kMaxNodeInc || ptr[j + 1] == ptr[j])) {
+                    err = "a macro node hosting a cell needs 1 to 12 incident elements";
+                    return nullptr;
+                }
             std::vector<unsigned char> dir(nn, 0);
             for (int e = 0; e < ne; ++e)
                 for (int p = 0; p < nv; ++p)
@@ -942,6 +988,7 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
     const size_t o_co = A.put(v.cell_offset, (size_t)v.n_cells);
     const size_t o_disc = v.max_discs > 0 ? A.put(v.cell_discs, (size_t)v.n_cells * v.max_discs * 3) : 0;
     const size_t o_Q = A.reserve<double>((size_t)v.n_cells);
+    const size_t o_de = A.reserve<unsigned char>((size_t)v.n_cells * nn_max);
     const size_t o_bad = A.reserve<int>((size_t)v.n_scen);
     const size_t o_T = A.reserve<DTmpl>(v.n_tmpl);
     const size_t o_S = A.reserve<DScen>(v.n_scen);
@@ -1011,7 +1058,18 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
     B.cell_tmpl = at<int>(base, o_ct); B.cell_scen = at<int>(base, o_cs); B.cell_off = at<long long>(base, o_co);
     B.discs = v.max_discs > 0 ? at<double>(base, o_disc) : nullptr;
     B.Q = at<double>(base, o_Q); B.bad = at<int>(base, o_bad);
+    B.dir_elem = at<unsigned char>(base, o_de);
     B.Ss = v.Ss; B.N = v.N;
+    {
+        const long long n = (long long)v.n_cells * nn_max;
+        if (n > 0) hipLaunchKernelGGL(k_dir_elem, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, B);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(0) != hipSuccess) {
+            err = "Dirichlet interpolation set-up kernel failed";
+            hipFree(P->buf);
+            delete P;
+            return nullptr;
+        }
+    }
     P->all_structured = !std::getenv("TSR_GENERIC_MICRO");
     for (int t = 0; t < v.n_tmpl; ++t) P->all_structured = P->all_structured && to[t].structured;
     P->lds = (size_t)4 * nn_max * sizeof(double) + (size_t)ne_max;
@@ -1106,6 +1164,8 @@ static hipError_t tune_micro(DevProblem* p, const GArgs& a, hipStream_t st)
             float ms = 0.0f;
             hipEventElapsedTime(&ms, e0, e1);
             const double avg = ms / 3.0;
+            if (std::getenv("TSR_TUNE_LOG"))
+                std::fprintf(stderr, "[tsr] k_micro mode %d: bs %d, waves/EU %d: %.1f us\n", mode, bs, w, avg * 1e3);
             if (mode == 0 && p->tune_n < 64) p->tune_ms[p->tune_n++] = avg;
             if (avg < best) { best = avg; best_bs = bs; best_w = w; }
         }
