@@ -165,8 +165,10 @@ double tsr_solver_last_gfunc_ms(const tsr_solver* s);
 /* Launch shape the autotuner chose for the micro-cell kernel on this problem
  * (workgroup size, waves-per-EU register budget; 0 before the first G). */
 void tsr_solver_micro_shape(const tsr_solver* s, int* bs, int* wpe);
-/* Average device time (ms) of each of G's three kernels (macro elements, micro cells,
- * macro nodes) over `reps` evaluations of G(u_dev), HIP events between the launches. */
+/* Average device time (ms) of each of G's three parts over `reps` evaluations of
+ * G(u_dev), HIP events between the launches: [0] is 0 (the macro elements run in the
+ * leading workgroups of the micro-cell launch), [1] the micro-cell launch, [2] the macro
+ * nodes. */
 int tsr_solver_time_gfunc_parts(tsr_solver* s, double* g_dev, const double* u_dev, int reps, double* ms3);
 /* As tsr_solver_time_gfunc_parts for one k_micro mode: 0 G, 1 G recording the
  * linearisation point, 2 an FD product with the linearised constitutive law (after a

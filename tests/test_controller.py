@@ -78,7 +78,8 @@ def test_fit_learns_a_smooth_map():
     rng = np.random.default_rng(4)
     X = rng.standard_normal((8000, 16)).astype(np.float32) * 3 + 1
     Y = np.stack([-4.0 + 0.3 * np.tanh(X[:, 0]) + 0.1 * X[:, 1], 8.0 + 2.0 * np.sin(X[:, 2])], 1).astype(np.float32)
-    model = CT.fit(X, Y, epochs=40)
+    # the oracle's own Adam on this data reaches 0.022 / 0.15 after 40 epochs at this rate
+    model = CT.fit(X, Y, epochs=40, lr=3e-3)
     print(f"[controller] fit: {model['val']}", flush=True)
     assert model["val"]["mae_log10_tau"] < 0.05
     assert model["val"]["mae_m"] < 0.5

@@ -73,8 +73,6 @@ struct DScen {
     const int* ptr;
     const int* inc;
     const unsigned char* dir;
-    double* alpha;                    // [4*ne] scratch: interpolation coefficients per macro element
-    double* hel;                      // [4*ne] scratch: vertex heads per macro element (this G call)
     double* qe;                       // [nv*ne] scratch
     double* bq;                       // [2*nv*ne] scratch
 };

@@ -1,13 +1,14 @@
 // gfunc.hip - the two-scale RHS G(u) on CDNA4 (gfx950), FP64.
 //
-// One G evaluation is three launches on one stream:
-//   k_macro_elem  one thread per macro element: macro interpolant coefficients, the
-//                 Keff-weighted sub-face fluxes and the BC half-edge inflows;
-//   k_micro       one workgroup per micro cell: node heads, element soils and the
-//                 node-centric van Genuchten evaluation in LDS, sub-face fluxes in
-//                 LDS, a deterministic node-centric gather, and the exchange term
-//                 (net inflow into the cell's non-Dirichlet CVs) by a wavefront
-//                 shuffle reduction;
+// One G evaluation is two launches on one stream:
+//   k_micro       leading workgroups, one thread per macro element: the Keff-weighted
+//                 sub-face fluxes and the BC half-edge inflows of the macro mesh;
+//                 then one workgroup per micro cell: node heads (the cell's Dirichlet
+//                 heads interpolated from the macro heads), element soils, the
+//                 node-centric van Genuchten evaluation in LDS, a deterministic
+//                 node-centric gather of sub-face fluxes, and the exchange term (net
+//                 inflow into the cell's non-Dirichlet CVs) by a wavefront shuffle
+//                 reduction;
 //   k_macro_node  one thread per macro node: gather of sub-face fluxes, BC inflows
 //                 and the exchange, divided by the macro storage.
 // The discrete model (geometry conventions, unknown layout, micro Dirichlet heads,
@@ -122,26 +123,35 @@ __device__ __forceinline__ void store_out(const DBatch& B, const GArgs& a, int s
 }
 
 // ---------------------------------------------------------------- macro elements
-__global__ __launch_bounds__(256) void k_macro_elem(DBatch B, GArgs a)
+// Vertex heads hv and interpolant coefficients al of macro element e (linear: al =
+// (a, b, c) of a x + b y + c; bilinear: (a, b, c, d) of a x + b y + c x y + d).
+__device__ __forceinline__ void macro_interp(const DScen& S, const GArgs& a, double ve, int e, double hv[4],
+                                             double al[4])
 {
-    const int s = blockIdx.y;
-    if (a.active && !a.active[s]) return;
-    const DScen& S = B.scen[s];
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= S.ne) return;
     const int nv = S.nv, ne = S.ne;
-    const double ve = a.v ? a.veps[s] : 0.0;
-    double hv[4] = {0, 0, 0, 0}, al[4] = {0, 0, 0, 0};
+    for (int i = 0; i < 4; ++i) { hv[i] = 0.0; al[i] = 0.0; }
     for (int i = 0; i < nv; ++i) hv[i] = macro_head(S, a, ve, S.el[i * ne + e]);
     for (int k = 0; k < nv; ++k) {
         double acc = 0.0;
         for (int i = 0; i < nv; ++i) acc += hv[i] * S.sf[(k * nv + i) * ne + e];
         al[k] = acc;
     }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) S.alpha[4 * e + k] = al[k];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) S.hel[4 * e + k] = hv[k];
+}
+
+// Workgroups of bs threads per scenario that the macro elements take in a k_micro launch.
+__host__ __device__ __forceinline__ int macro_wgs(const DBatch& B, int bs) { return (B.macro_ne_max + bs - 1) / bs; }
+
+// Macro element e of scenario s: sub-face fluxes qe (Keff at the sub-face midpoints) and
+// boundary-edge fluxes bq, gathered per node by k_macro_node.  Runs in the leading
+// workgroups of the k_micro launch (the micro cells do not depend on it).
+__device__ __forceinline__ void macro_elem(const DBatch& B, const GArgs& a, int s, int e)
+{
+    const DScen& S = B.scen[s];
+    if (e >= S.ne) return;
+    const int nv = S.nv, ne = S.ne;
+    const double ve = a.v ? a.veps[s] : 0.0;
+    double hv[4], al[4];
+    macro_interp(S, a, ve, e, hv, al);
     for (int j = 0; j < nv; ++j) {
         const double x = S.midx[j * ne + e], y = S.midy[j * ne + e];
         double gx, gy, hip;
@@ -456,7 +466,15 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
     __shared__ double s_mel[kMaxNodeInc][12];
     __shared__ double s_part[256];
     __shared__ double s_geo[2][22];
-    const int cell = blockIdx.x;
+    // the leading workgroups of the launch evaluate the macro elements (one thread each,
+    // every scenario); the rest are the micro cells
+    const int mwg = macro_wgs(B, blockDim.x);
+    if ((int)blockIdx.x < mwg * B.n_scen) {
+        const int sm = blockIdx.x / mwg;
+        if (!a.active || a.active[sm]) macro_elem(B, a, sm, (blockIdx.x % mwg) * blockDim.x + threadIdx.x);
+        return;
+    }
+    const int cell = blockIdx.x - mwg * B.n_scen;
     const int s = B.cell_scen[cell];
     if (a.active && !a.active[s]) return;
     const int t = B.cell_tmpl[cell];
@@ -494,7 +512,8 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
     // host interpolation data
     if (!node_host) {
         if (tid == 0) {
-            const double* al = S.alpha + 4 * host;
+            double hv[4], al[4];
+            macro_interp(S, a, ve, host, hv, al);
             if (S.nv == 3) { s_bc[0] = al[0]; s_bc[1] = al[1]; s_bc[2] = 0.0; s_bc[3] = al[2]; }
             else { s_bc[0] = al[0]; s_bc[1] = al[1]; s_bc[2] = al[2]; s_bc[3] = al[3]; }
             s_bc[4] = S.cen[host];
@@ -512,7 +531,7 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
 #pragma unroll
             for (int k = 0; k < 9; ++k) s_mel[tid][k] = S.sf[k * S.ne + e];
 #pragma unroll
-            for (int v = 0; v < 3; ++v) s_mel[tid][9 + v] = S.hel[4 * e + v];   // written by k_macro_elem
+            for (int v = 0; v < 3; ++v) s_mel[tid][9 + v] = macro_head(S, a, ve, S.el[v * S.ne + e]);
         }
     }
     if constexpr (STRUCT) {
@@ -878,7 +897,7 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
     Arena A;
     struct TOff { size_t ptr, inc, el, sf, nrm, w, scv, vec, cen, bnd, soil, blist; int nn, ne; double ca; bool has_soil;
                   int structured, r, nbnd; unsigned mul_r, mul_rm; double dx = 0.0; double sg[2][22]; };
-    struct SOff { size_t keff, el, nodes, sf, nrm, midx, midy, cv, scv, bce, blen, bmid, bnrm, cen, ptr, inc, dir, alpha, hel, qe, bq; };
+    struct SOff { size_t keff, el, nodes, sf, nrm, midx, midy, cv, scv, bce, blen, bmid, bnrm, cen, ptr, inc, dir, qe, bq; };
     std::vector<TOff> to(v.n_tmpl);
     std::vector<SOff> so(v.n_scen);
     int nn_max = 0, ne_max = 0, mnn = 0, mne = 0;
@@ -974,8 +993,6 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
             o = r;
             o.keff = keep;
         }
-        o.alpha = A.reserve<double>((size_t)4 * ne);
-        o.hel = A.reserve<double>((size_t)4 * ne);
         o.qe = A.reserve<double>((size_t)nv * ne);
         o.bq = A.reserve<double>((size_t)2 * nv * ne);
     }
@@ -1042,7 +1059,7 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
         S.be_len = at<double>(base, o.blen); S.be_mid = at<double>(base, o.bmid); S.be_nrm = at<double>(base, o.bnrm);
         S.cen = at<double>(base, o.cen); S.ptr = at<int>(base, o.ptr); S.inc = at<int>(base, o.inc);
         S.dir = at<unsigned char>(base, o.dir);
-        S.alpha = at<double>(base, o.alpha); S.hel = at<double>(base, o.hel); S.qe = at<double>(base, o.qe); S.bq = at<double>(base, o.bq);
+        S.qe = at<double>(base, o.qe); S.bq = at<double>(base, o.bq);
         std::memcpy(A.host.data() + o_S + s * sizeof(DScen), &S, sizeof(DScen));
     }
     if (hipMemcpy(P->buf, A.host.data(), P->bytes, hipMemcpyHostToDevice) != hipSuccess) {
@@ -1130,7 +1147,8 @@ static void launch_micro(const DevProblem* p, const GArgs& a, int bs, int wpe, h
 {
     const DBatch& B = p->B;
     MicroFn f = micro_fn(p->all_structured, micro_mode(a), wpe);
-    hipLaunchKernelGGL(f, dim3(B.n_cells), dim3(bs), micro_mode(a) == 2 ? p->lds2 : p->lds, st, B, a);
+    const int grid = B.n_cells + macro_wgs(B, bs) * B.n_scen;
+    hipLaunchKernelGGL(f, dim3(grid), dim3(bs), micro_mode(a) == 2 ? p->lds2 : p->lds, st, B, a);
 }
 
 // Autotuner: on the first G of a problem (per k_micro mode), time every admissible
@@ -1181,10 +1199,7 @@ hipError_t launch_gfunc_part(const DevProblem* p, const GArgs& a, int part, hipS
 {
     const DBatch& B = p->B;
     if (part == 0) {
-        // 64-thread workgroups: the macro meshes are small (10^4-10^5 elements), so
-        // spread them over as many CUs as possible
-        dim3 ge((B.macro_ne_max + 63) / 64, B.n_scen);
-        hipLaunchKernelGGL(k_macro_elem, ge, dim3(64), 0, st, B, a);
+        // nothing: the macro elements run in the leading workgroups of the micro launch
     } else if (part == 1) {
         const int mode = micro_mode(a);
         if (mode == 2 && !lin_mode_ok(p)) return hipErrorInvalidValue;
