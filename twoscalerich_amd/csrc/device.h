@@ -92,6 +92,10 @@ struct DBatch {
     // index (into the host node's incident macro elements) of the element whose P1
     // interpolant gives the node's head; fixed by the geometry, filled once at upload
     unsigned char* dir_elem;
+    // element soils of every cell (bit e of the cell's soil_words words: 1 = soil A),
+    // from the template or the cell's inclusion discs; fixed, filled once at upload
+    unsigned* soil_bits;
+    int soil_words;
     double Ss;
     long long N;
 };

@@ -251,6 +251,21 @@ __device__ __forceinline__ int elem_soil(const DBatch& B, const DTmpl& T, int ce
     return T.soil ? (int)T.soil[e] : TSR_SOIL_B;
 }
 
+// Upload-time soil map: one thread per (cell, 32-element word).
+__global__ __launch_bounds__(256) void k_soil_bits(DBatch B)
+{
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)B.n_cells * B.soil_words) return;
+    const int cell = (int)(id / B.soil_words), w = (int)(id % B.soil_words);
+    const DTmpl& T = B.tmpl[B.cell_tmpl[cell]];
+    unsigned bits = 0u;
+    for (int k = 0; k < 32; ++k) {
+        const int e = 32 * w + k;
+        if (e < T.ne && elem_soil(B, T, cell, e) == TSR_SOIL_A) bits |= 1u << k;
+    }
+    B.soil_bits[id] = bits;
+}
+
 // Visit the incident (element, local vertex) pairs of node i in ascending element
 // order (the order the oracle's CSR gives).  Generic templates read the CSR; the
 // structured grid calls f on the (at most six) pairs of node (ix, iy) in straight-line
@@ -607,7 +622,11 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
         for (int i = tid; i < nn; i += bs)
             if (!is_dirichlet<STRUCT>(T, i)) h[i] = ld_state(a, ve, off + i);
     }
-    for (int e = tid; e < ne; e += bs) soil[e] = (unsigned char)elem_soil(B, T, cell, e);
+    {
+        const unsigned* sb = B.soil_bits + (size_t)cell * B.soil_words;
+#pragma unroll 1
+        for (int e = tid; e < ne; e += bs) soil[e] = ((sb[e >> 5] >> (e & 31)) & 1u) ? TSR_SOIL_A : TSR_SOIL_B;
+    }
     __syncthreads();
 
     // node-centric constitutive evaluation (MODE 0/1)
@@ -1006,6 +1025,8 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
     const size_t o_disc = v.max_discs > 0 ? A.put(v.cell_discs, (size_t)v.n_cells * v.max_discs * 3) : 0;
     const size_t o_Q = A.reserve<double>((size_t)v.n_cells);
     const size_t o_de = A.reserve<unsigned char>((size_t)v.n_cells * nn_max);
+    const int soil_words = (ne_max + 31) / 32;
+    const size_t o_sb = A.reserve<unsigned>((size_t)v.n_cells * soil_words);
     const size_t o_bad = A.reserve<int>((size_t)v.n_scen);
     const size_t o_T = A.reserve<DTmpl>(v.n_tmpl);
     const size_t o_S = A.reserve<DScen>(v.n_scen);
@@ -1076,12 +1097,15 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
     B.discs = v.max_discs > 0 ? at<double>(base, o_disc) : nullptr;
     B.Q = at<double>(base, o_Q); B.bad = at<int>(base, o_bad);
     B.dir_elem = at<unsigned char>(base, o_de);
+    B.soil_bits = at<unsigned>(base, o_sb);
+    B.soil_words = soil_words;
     B.Ss = v.Ss; B.N = v.N;
     {
-        const long long n = (long long)v.n_cells * nn_max;
+        const long long n = (long long)v.n_cells * nn_max, nw = (long long)v.n_cells * soil_words;
         if (n > 0) hipLaunchKernelGGL(k_dir_elem, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, B);
+        if (nw > 0) hipLaunchKernelGGL(k_soil_bits, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, 0, B);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(0) != hipSuccess) {
-            err = "Dirichlet interpolation set-up kernel failed";
+            err = "set-up kernels (Dirichlet interpolation, soil map) failed";
             hipFree(P->buf);
             delete P;
             return nullptr;
