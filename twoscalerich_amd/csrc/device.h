@@ -96,6 +96,16 @@ struct DBatch {
     // from the template or the cell's inclusion discs; fixed, filled once at upload
     unsigned* soil_bits;
     int soil_words;
+    // node-hosted cells, per incident macro element k < kMaxNodeInc of the host node:
+    // shape-function coefficients (cm_sf[(c * kMaxNodeInc + k) * 9 + j], the macro
+    // mesh's shape_funcs of that element) and the state index of each vertex's head
+    // (cm_vi[(c * kMaxNodeInc + k) * 3 + v], -1 at Dirichlet macro nodes); cm_host[c]
+    // the host node's state index (-1: Dirichlet), cm_ninc[c] its number of elements.
+    // Geometry only, filled once at upload, so a cell's set-up is one load deep.
+    double* cm_sf;
+    int* cm_vi;
+    int* cm_host;
+    unsigned char* cm_ninc;
     double Ss;
     long long N;
 };

@@ -108,6 +108,12 @@ __device__ __forceinline__ double macro_head(const DScen& S, const GArgs& a, dou
     return S.dir[j] ? S.dirichlet_head : ld_state(a, ve, S.u_off + j);
 }
 
+// Head at state index gi (-1: a Dirichlet macro node, head dh).
+__device__ __forceinline__ double state_or(const GArgs& a, double ve, int gi, double dh)
+{
+    return gi < 0 ? dh : ld_state(a, ve, gi);
+}
+
 __device__ __forceinline__ double wave_sum(double x)
 {
 #pragma unroll
@@ -231,6 +237,37 @@ __global__ __launch_bounds__(256) void k_dir_elem(DBatch B)
         }
     }
     B.dir_elem[id] = (unsigned char)best;
+}
+
+// Upload-time copy of each node-hosted cell's macro interpolation data (one thread
+// per (cell, incident element slot)).
+__global__ __launch_bounds__(256) void k_cell_macro(DBatch B)
+{
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)B.n_cells * kMaxNodeInc) return;
+    const int cell = (int)(id / kMaxNodeInc), k = (int)(id % kMaxNodeInc);
+    const DScen& S = B.scen[B.cell_scen[cell]];
+    double* sf = B.cm_sf + id * 9;
+    int* vi = B.cm_vi + id * 3;
+    for (int j = 0; j < 9; ++j) sf[j] = 0.0;
+    for (int v = 0; v < 3; ++v) vi[v] = -1;
+    if (S.host != TSR_HOST_NODE) {
+        if (k == 0) { B.cm_host[cell] = -1; B.cm_ninc[cell] = 0; }
+        return;
+    }
+    const int host = cell - S.first_cell;
+    const int q0 = S.ptr[host], ninc = S.ptr[host + 1] - q0;
+    if (k == 0) {
+        B.cm_host[cell] = S.dir[host] ? -1 : (int)(S.u_off + host);
+        B.cm_ninc[cell] = (unsigned char)ninc;
+    }
+    if (k >= ninc) return;
+    const int e = S.inc[q0 + k] >> 2;
+    for (int j = 0; j < 9; ++j) sf[j] = S.sf[j * S.ne + e];
+    for (int v = 0; v < 3; ++v) {
+        const int n = S.el[v * S.ne + e];
+        vi[v] = S.dir[n] ? -1 : (int)(S.u_off + n);
+    }
 }
 
 // ---------------------------------------------------------------- micro cells
@@ -535,18 +572,19 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
             s_bc[5] = S.cen[S.ne + host];
         }
     } else {
-        const int q0 = S.ptr[host], ninc = S.ptr[host + 1] - q0;
+        const int ninc = B.cm_ninc[cell];
         if (tid == 0) {
-            s_bc[3] = macro_head(S, a, ve, host);
+            s_bc[3] = state_or(a, ve, B.cm_host[cell], S.dirichlet_head);
             s_bc[4] = S.nodes[host];
             s_bc[5] = S.nodes[S.nn + host];
         }
         if (model != 1 && tid < ninc) {
-            const int e = S.inc[q0 + tid] >> 2;
+            const double* sf = B.cm_sf + ((size_t)cell * kMaxNodeInc + tid) * 9;
+            const int* vi = B.cm_vi + ((size_t)cell * kMaxNodeInc + tid) * 3;
 #pragma unroll
-            for (int k = 0; k < 9; ++k) s_mel[tid][k] = S.sf[k * S.ne + e];
+            for (int k = 0; k < 9; ++k) s_mel[tid][k] = sf[k];
 #pragma unroll
-            for (int v = 0; v < 3; ++v) s_mel[tid][9 + v] = macro_head(S, a, ve, S.el[v * S.ne + e]);
+            for (int v = 0; v < 3; ++v) s_mel[tid][9 + v] = state_or(a, ve, vi[v], S.dirichlet_head);
         }
     }
     if constexpr (STRUCT) {
@@ -1027,10 +1065,15 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
     const size_t o_de = A.reserve<unsigned char>((size_t)v.n_cells * nn_max);
     const int soil_words = (ne_max + 31) / 32;
     const size_t o_sb = A.reserve<unsigned>((size_t)v.n_cells * soil_words);
+    const size_t o_cmsf = A.reserve<double>((size_t)v.n_cells * kMaxNodeInc * 9);
+    const size_t o_cmvi = A.reserve<int>((size_t)v.n_cells * kMaxNodeInc * 3);
+    const size_t o_cmh = A.reserve<int>((size_t)v.n_cells);
+    const size_t o_cmn = A.reserve<unsigned char>((size_t)v.n_cells);
     const size_t o_bad = A.reserve<int>((size_t)v.n_scen);
     const size_t o_T = A.reserve<DTmpl>(v.n_tmpl);
     const size_t o_S = A.reserve<DScen>(v.n_scen);
 
+    if (v.N >= 0x7fffffffLL) { err = "more than 2^31 - 1 unknowns"; return nullptr; }
     DevProblem* P = new DevProblem();
     P->bytes = A.host.size();
     if (hipMalloc(&P->buf, P->bytes) != hipSuccess) { err = "hipMalloc failed for the problem"; delete P; return nullptr; }
@@ -1099,13 +1142,17 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
     B.dir_elem = at<unsigned char>(base, o_de);
     B.soil_bits = at<unsigned>(base, o_sb);
     B.soil_words = soil_words;
+    B.cm_sf = at<double>(base, o_cmsf); B.cm_vi = at<int>(base, o_cmvi);
+    B.cm_host = at<int>(base, o_cmh); B.cm_ninc = at<unsigned char>(base, o_cmn);
     B.Ss = v.Ss; B.N = v.N;
     {
         const long long n = (long long)v.n_cells * nn_max, nw = (long long)v.n_cells * soil_words;
         if (n > 0) hipLaunchKernelGGL(k_dir_elem, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, B);
         if (nw > 0) hipLaunchKernelGGL(k_soil_bits, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, 0, B);
+        const long long nk = (long long)v.n_cells * kMaxNodeInc;
+        if (nk > 0) hipLaunchKernelGGL(k_cell_macro, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, 0, B);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(0) != hipSuccess) {
-            err = "set-up kernels (Dirichlet interpolation, soil map) failed";
+            err = "set-up kernels (Dirichlet interpolation, soil map, cell macro data) failed";
             hipFree(P->buf);
             delete P;
             return nullptr;
@@ -1193,11 +1240,22 @@ static hipError_t tune_micro(DevProblem* p, const GArgs& a, hipStream_t st)
     if (cand_bs.empty()) cand_bs.push_back(kMicroMaxBS);
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return hipErrorUnknown;
+    // register budgets whose code spills to scratch are not candidates: besides being
+    // slow, a spilling build of this kernel (mode 1, 8 waves per SIMD) has given
+    // wrong results
+    std::vector<int> cand_w;
+    for (int w : kWpe) {
+        hipFuncAttributes fa;
+        if (hipFuncGetAttributes(&fa, (const void*)micro_fn(p->all_structured, mode, w)) == hipSuccess &&
+            fa.localSizeBytes == 0)
+            cand_w.push_back(w);
+    }
+    if (cand_w.empty()) cand_w.push_back(1);
     double best = 1e300;
-    int best_bs = cand_bs[0], best_w = 1;
+    int best_bs = cand_bs[0], best_w = cand_w[0];
     if (mode == 0) p->tune_n = 0;
     for (int bs : cand_bs)
-        for (int w : kWpe) {
+        for (int w : cand_w) {
             launch_micro(p, a, bs, w, st);
             hipEventRecord(e0, st);
             for (int r = 0; r < 3; ++r) launch_micro(p, a, bs, w, st);
