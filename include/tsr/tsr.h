@@ -4,8 +4,9 @@
  * Three groups of entry points:
  *   1. Reference signatures, so a caller written for the reference links unchanged.
  *      Each cites the reference declaration it replaces.
- *   2. Problem set-up: the BASELINE configurations and the reference's own input files,
- *      producing the batch view (tsr_types.h) that the device path consumes.
+ *   2. Problem set-up: the benchmark cases (small, plain, ragged, 64-scenario, large)
+ *      and the reference's own input files, producing the batch view (tsr_types.h)
+ *      that the device path consumes.
  *   3. A device-resident solver: state stays in HBM between steps.
  *
  * Errors: functions returning int give 0 on success; nonzero codes are listed per

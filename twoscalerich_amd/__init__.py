@@ -2,7 +2,7 @@
 
 Host-side handles over libtsr.so (HIP kernels for gfx950 + C++ host code):
 
-    Problem   a workload (BASELINE configurations or the reference's input files) as a
+    Problem   a workload (a benchmark case or the reference's input files) as a
               batch view, with the reference's user_data_struct for drop-in calls;
     Solver    the device-resident EXPREM integrator and the GPU right-hand side.
 

@@ -1,6 +1,7 @@
-// problem.cpp - the workloads (BASELINE configurations and the reference's own input
-// files) assembled into a batch view, plus the registry that lets reference-signature
-// entry points find the view of a user_data_struct.
+// problem.cpp - the workloads (the small, plain, ragged, 64-scenario and large benchmark
+// cases, and the reference's own input files) assembled into a batch view, plus the
+// registry that lets reference-signature entry points find the view of a
+// user_data_struct.
 #include "tsr_internal.h"
 
 #include <cmath>
