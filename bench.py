@@ -24,12 +24,15 @@ Extra fields on the JSON line:
                 plain path's full-horizon time from profiles/ for reference.
   gfunc         G evaluation times per kernel for the three micro-kernel modes.
   roofline      the dominant kernel of the timed run (k_micro in its linearised mode, the
-                FD products): algorithmic bytes of one launch (80 B per interior micro
-                node: u, v, 6 expansion components, g0, out; 56 B per Dirichlet node:
-                recorded head, 4 conductivity components, g0, out) over
-                its average duration from HIP events on the solver stream; peak 8 TB/s;
-                traffic = FETCH+WRITE bytes per launch from rocprofv3 counters (profiles/),
-                scaled by the copy-kernel calibration, or null.
+                FD products Jv = (G(u + eps v) - G(u)) / eps): algorithmic bytes of one
+                launch = those of an FD product with the perturbation fused into the
+                state load, 32 B per unknown (read u, v and G(u), write the product)
+                plus the distinct per-cell / per-scenario data, over its average
+                duration from HIP events on the solver stream; peak 8 TB/s.
+                traffic = FETCH+WRITE bytes per launch from rocprofv3 counters
+                (profiles/), scaled by the copy-kernel calibration, or null; it exceeds
+                the algorithmic bytes because the linearised form also streams the
+                six recorded expansion components per micro node.
   cpu_baseline  the CPU oracle (FP64 C++/OpenMP restatement of the plain algorithm,
                 "port") timed on this host: EXPREM steps of the same workload for a
                 bounded sample, scaled to t_end by the plain GPU path's time profile.
@@ -175,7 +178,8 @@ def main():
     n_cells = v.n_cells
     n_dir = n_cells * (4 * (int(round(r ** 0.5)) - 1))
     n_int = n_micro - n_dir
-    lin_bytes = 80.0 * n_int + 56.0 * n_dir + p.distinct_bytes()
+    lin_bytes = 32.0 * p.N + p.distinct_bytes()
+    lin_impl_bytes = 80.0 * n_int + 56.0 * n_dir + p.distinct_bytes()   # what the linearised form reads and writes
     lin_ms = gf["linearised_fd"]["micro_ms"]
     achieved = lin_bytes / (lin_ms * 1e-3) / 1e9
     full_bytes = 16.0 * p.N + p.distinct_bytes()
@@ -233,7 +237,8 @@ def main():
         "gfunc": gf,
         "roofline": {"kernel": "k_micro (linearised FD product)", "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "algorithmic_bytes": lin_bytes, "avg_launch_ms": lin_ms,
+                     "traffic": traffic, "algorithmic_bytes": lin_bytes, "implementation_bytes": lin_impl_bytes,
+                     "avg_launch_ms": lin_ms,
                      "copy_kernel_GB_per_s": round(copy_gbs, 1)},
         "cpu_baseline": cpu,
     }

@@ -320,7 +320,8 @@ def dataset_from_arrays(d, **kw):
     Xs, Ys = [], []
     for q in np.unique(seq):
         sel = seq == q
-        cols = {k: Fd[sel, i] for i, k in enumerate(REC_FIELDS)}
+        # files written before later fields were added hold a prefix of REC_FIELDS
+        cols = {k: Fd[sel, i] for i, k in enumerate(REC_FIELDS[:Fd.shape[1]])}
         Y, good = label_sequence(cols, **kw)
         Xs.append(X[sel][good])
         Ys.append(Y[good])
@@ -384,6 +385,7 @@ def main(argv=None):
     ap.add_argument("--tend", type=float, default=0.0)
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--max-samples", type=int, default=60000)
     a = ap.parse_args(argv)
     if a.summarize:
@@ -403,10 +405,10 @@ def main(argv=None):
             keep = rng.choice(len(X), a.max_samples, replace=False)
             X, Y = X[keep], Y[keep]
         t0 = time.time()
-        model = fit(X, Y, epochs=a.epochs)
+        model = fit(X, Y, epochs=a.epochs, lr=a.lr)
         print(f"[controller] trained on {len(X)} samples in {time.time() - t0:.1f} s; loss "
               f"{model['hist'][0]:.4f} -> {model['hist'][-1]:.4f}; validation {model['val']}", flush=True)
-        save(model, a.out, n_samples=len(X), epochs=a.epochs)
+        save(model, a.out, n_samples=len(X), epochs=a.epochs, lr=a.lr)
 
 
 if __name__ == "__main__":
