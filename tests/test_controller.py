@@ -71,6 +71,27 @@ def test_records_by_scenario_drops_finished():
     assert [[r.tau for r in q] for q in seqs] == [[1.0, 3.0, 5.0], [2.0]]
 
 
+def test_dataset_from_arrays_reads_record_files_with_fewer_fields():
+    # a records file written before the last field existed holds a prefix of the fields;
+    # it labels the same as the full layout (the labels use tau_first, err_first, m, mD)
+    tau, m = [], []
+    for cyc in range(4):
+        t = 1.0
+        for mk in (6, 13, 19, 29):
+            tau.append(t)
+            m.append(mk)
+            t *= 1.25
+    cols = _cols(tau, m)
+    n = len(tau)
+    full = np.stack([cols[k] for k in CT.REC_FIELDS], 1)
+    X = np.arange(n * CT.N_IN, dtype=np.float32).reshape(n, CT.N_IN)
+    seq = np.zeros(n, np.int64)
+    Xa, Ya = CT.dataset_from_arrays({"X": X, "fields": full, "seq": seq}, window=20)
+    Xb, Yb = CT.dataset_from_arrays({"X": X, "fields": full[:, :-1], "seq": seq}, window=20)
+    assert np.array_equal(Xa, Xb) and np.array_equal(Ya, Yb)
+    assert np.allclose(Ya[:, 1], 6.0)
+
+
 # ------------------------------------------------------------------ GPU
 
 @pytest.mark.gpu
