@@ -18,6 +18,7 @@
 // Every kernel reads the state as u + veps*v (the finite-difference perturbation of
 // the Krylov iteration, never materialised) and can write (G - g0) * ieps directly.
 #include "device.h"
+#include "wave.h"
 
 #include <algorithm>
 #include <cmath>
@@ -114,12 +115,7 @@ __device__ __forceinline__ double state_or(const GArgs& a, double ve, int gi, do
     return gi < 0 ? dh : ld_state(a, ve, gi);
 }
 
-__device__ __forceinline__ double wave_sum(double x)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    return x;
-}
+__device__ __forceinline__ double wave_sum(double x) { return wave_sum_dpp(x); }
 
 __device__ __forceinline__ void store_out(const DBatch& B, const GArgs& a, int s, long long i, double g)
 {
