@@ -23,17 +23,55 @@ namespace {
 // the CPU has it) and for baseline x86-64.
 #define TSR_HOST_CLONES __attribute__((target_clones("arch=x86-64-v3", "default")))
 
-// C <- alpha * A * B + beta * C, n x n column-major.
+// C <- alpha * A * B + beta * C, n x n column-major.  Each element of A B is summed over k
+// in increasing order (a zero entry of B adds a zero product, which changes no finite
+// sum); blocks of 8 rows x 4 columns keep their 32 sums in registers, so every loaded
+// element of A serves four columns.
+typedef double v4d __attribute__((vector_size(32)));
+#define TSR_LOAD4(v, p) std::memcpy(&(v), (p), sizeof(v4d))
+#define TSR_STORE4(p, v) std::memcpy((p), &(v), sizeof(v4d))
+
 TSR_HOST_CLONES void mm(int n, double alpha, const double* A, const double* B, double beta, double* C, double* tmp)
 {
-    for (int j = 0; j < n; ++j) {
-        double* t = tmp + (size_t)j * n;
-        for (int i = 0; i < n; ++i) t[i] = 0.0;
-        for (int k = 0; k < n; ++k) {
-            const double b = B[k + (size_t)j * n];
-            if (b == 0.0) continue;
-            const double* a = A + (size_t)k * n;
-            for (int i = 0; i < n; ++i) t[i] += a[i] * b;
+    const int n8 = n & ~7, n4 = n & ~3;
+    for (int j0 = 0; j0 < n4; j0 += 4) {
+        const double* b0 = B + (size_t)j0 * n;
+        const double* b1 = b0 + n;
+        const double* b2 = b1 + n;
+        const double* b3 = b2 + n;
+        for (int i0 = 0; i0 < n8; i0 += 8) {
+            v4d c00 = {0, 0, 0, 0}, c10 = c00, c01 = c00, c11 = c00, c02 = c00, c12 = c00, c03 = c00, c13 = c00;
+            for (int k = 0; k < n; ++k) {
+                const double* a = A + (size_t)k * n + i0;
+                v4d a0, a1;
+                TSR_LOAD4(a0, a);
+                TSR_LOAD4(a1, a + 4);
+                const double s0 = b0[k], s1 = b1[k], s2 = b2[k], s3 = b3[k];
+                c00 = c00 + a0 * s0; c10 = c10 + a1 * s0;
+                c01 = c01 + a0 * s1; c11 = c11 + a1 * s1;
+                c02 = c02 + a0 * s2; c12 = c12 + a1 * s2;
+                c03 = c03 + a0 * s3; c13 = c13 + a1 * s3;
+            }
+            double* t = tmp + (size_t)j0 * n + i0;
+            TSR_STORE4(t, c00); TSR_STORE4(t + 4, c10);
+            TSR_STORE4(t + n, c01); TSR_STORE4(t + n + 4, c11);
+            TSR_STORE4(t + 2 * n, c02); TSR_STORE4(t + 2 * n + 4, c12);
+            TSR_STORE4(t + 3 * n, c03); TSR_STORE4(t + 3 * n + 4, c13);
+        }
+        for (int i = n8; i < n; ++i)
+            for (int c = 0; c < 4; ++c) {
+                const double* bc = b0 + (size_t)c * n;
+                double acc = 0.0;
+                for (int k = 0; k < n; ++k) acc = acc + A[i + (size_t)k * n] * bc[k];
+                tmp[i + (size_t)(j0 + c) * n] = acc;
+            }
+    }
+    for (int j = n4; j < n; ++j) {
+        const double* bj = B + (size_t)j * n;
+        for (int i = 0; i < n; ++i) {
+            double acc = 0.0;
+            for (int k = 0; k < n; ++k) acc = acc + A[i + (size_t)k * n] * bj[k];
+            tmp[i + (size_t)j * n] = acc;
         }
     }
     const size_t nn = (size_t)n * n;
@@ -52,8 +90,9 @@ double inf_norm(int n, const double* A)
     return r;
 }
 
-// Gaussian elimination with partial pivoting; X overwrites B (n x nrhs).
-TSR_HOST_CLONES bool solve(int n, double* A, double* B, int nrhs)
+// Gaussian elimination with partial pivoting; X overwrites B (n x nrhs); R: n * nrhs
+// doubles of workspace.
+TSR_HOST_CLONES bool solve(int n, double* A, double* B, int nrhs, double* R)
 {
     int p[256];   // n <= 2 * kMaxKrylovCap + 3 (augmented matrices)
     if (n > 256) return false;
@@ -76,15 +115,34 @@ TSR_HOST_CLONES bool solve(int n, double* A, double* B, int nrhs)
             for (int i = k + 1; i < n; ++i) A[i + (size_t)j * n] -= A[i + (size_t)k * n] * f;
         }
     }
-    for (int r = 0; r < nrhs; ++r) {
-        double* b = B + (size_t)r * n;
-        for (int k = 0; k < n; ++k) if (p[k] != k) std::swap(b[k], b[p[k]]);
-        for (int k = 0; k < n; ++k) for (int i = k + 1; i < n; ++i) b[i] -= A[i + (size_t)k * n] * b[k];
-        for (int k = n - 1; k >= 0; --k) {
-            b[k] /= A[k + (size_t)k * n];
-            for (int i = 0; i < k; ++i) b[i] -= A[i + (size_t)k * n] * b[k];
+    // the right-hand sides row by row (row i of every column contiguous), so the
+    // substitutions run across the columns; each element sees the same operations in the
+    // same order as in a column-by-column solve
+    for (int r = 0; r < nrhs; ++r)
+        for (int i = 0; i < n; ++i) R[(size_t)i * nrhs + r] = B[i + (size_t)r * n];
+    for (int k = 0; k < n; ++k)
+        if (p[k] != k)
+            for (int r = 0; r < nrhs; ++r) std::swap(R[(size_t)k * nrhs + r], R[(size_t)p[k] * nrhs + r]);
+    for (int k = 0; k < n; ++k) {
+        const double* rk = &R[(size_t)k * nrhs];
+        for (int i = k + 1; i < n; ++i) {
+            const double a = A[i + (size_t)k * n];
+            double* ri = &R[(size_t)i * nrhs];
+            for (int r = 0; r < nrhs; ++r) ri[r] -= a * rk[r];
         }
     }
+    for (int k = n - 1; k >= 0; --k) {
+        double* rk = &R[(size_t)k * nrhs];
+        const double d = A[k + (size_t)k * n];
+        for (int r = 0; r < nrhs; ++r) rk[r] /= d;
+        for (int i = 0; i < k; ++i) {
+            const double a = A[i + (size_t)k * n];
+            double* ri = &R[(size_t)i * nrhs];
+            for (int r = 0; r < nrhs; ++r) ri[r] -= a * rk[r];
+        }
+    }
+    for (int r = 0; r < nrhs; ++r)
+        for (int i = 0; i < n; ++i) B[i + (size_t)r * n] = R[(size_t)i * nrhs + r];
     return true;
 }
 
@@ -115,7 +173,7 @@ int phi1_pade(double* phiH, const double* H, double alpha, int m, int ldh, int l
         if (k > 0) { mm(m, 1.0, P, A, 0.0, T, tmp.data()); std::memcpy(P, T, n2 * sizeof(double)); }
         for (size_t i = 0; i < n2; ++i) { N[i] += num[k] * P[i]; D[i] += den[k] * P[i]; }
     }
-    if (!solve(m, D, N, m)) return 1;
+    if (!solve(m, D, N, m, tmp.data())) return 1;
     if (s > 0) {
         mm(m, 1.0, A, N, 1.0, E, tmp.data());
         std::memcpy(T, N, n2 * sizeof(double));
@@ -155,7 +213,7 @@ int expm_pade6(double* E, const double* A0, int n)
         const double sg = (k & 1) ? -1.0 : 1.0;
         for (size_t i = 0; i < n2; ++i) { N[i] += c[k] * P[i]; D[i] += sg * c[k] * P[i]; }
     }
-    if (!solve(n, D, N, n)) return 1;
+    if (!solve(n, D, N, n, tmp.data())) return 1;
     for (int k = 0; k < s; ++k) { mm(n, 1.0, N, N, 0.0, T, tmp.data()); std::memcpy(N, T, n2 * sizeof(double)); }
     std::memcpy(E, N, n2 * sizeof(double));
     return 0;
