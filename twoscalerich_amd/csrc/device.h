@@ -180,6 +180,10 @@ hipError_t launch_lincomb(double* out, const double* base, const double* const* 
                           const double* coef, int ldc, double sgn, const BlkMap& M, int nblk, double* part, int ldp,
                           int partmode, const double* u, double at, double rt, const double* extra_sub,
                           const int* active, hipStream_t st);
+// out1 = base1 + sum_{k < nk1[s]} coef1 col_k, out2 = sum_{k < nk2[s]} coef2 col_k, one pass
+hipError_t launch_lincomb2(double* out1, const double* base1, double* out2, const double* const* cols,
+                           const int* nk1_s, const double* coef1, const int* nk2_s, const double* coef2, int ldc,
+                           const BlkMap& M, int nblk, const int* active, hipStream_t st);
 hipError_t launch_reduce(int mode, const double* part, int ldp, int nk, const int* blk_first, int n_scen,
                          const long long* scen_rows, KrylovState K, int j, const int* active, hipStream_t st);
 // One Arnoldi orthogonalisation by DCGS2 (nk = j + 1 columns; w becomes raw column j+1

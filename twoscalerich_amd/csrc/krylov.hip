@@ -430,6 +430,55 @@ __global__ __launch_bounds__(kVecBS) void k_lincomb(double* out, const double* b
     }
 }
 
+// out1 = base1 + sum_{k < nk1} coef1_k col_k and out2 = sum_{k < nk2} coef2_k col_k in one
+// pass over the columns: the two combinations of an EXPREM step that share the basis
+// (U2 and the Arnoldi-relation term of D).  Each sum runs over increasing k as in
+// k_lincomb, so both outputs equal two separate k_lincomb calls bit for bit.
+__global__ __launch_bounds__(kVecBS) void k_lincomb2(double* out1, const double* base1, double* out2,
+                                                    const double* const* cols, const int* nk1_s, const double* coef1,
+                                                    const int* nk2_s, const double* coef2, int ldc, BlkMap M,
+                                                    const int* active)
+{
+    const int b = blockIdx.x;
+    const int s = M.scen[b];
+    if (active && !active[s]) return;
+    const long long r0 = M.r0[b], r1 = M.r1[b];
+    const int nk1 = nk1_s[s], nk2 = nk2_s[s];
+    const int nk = nk1 > nk2 ? nk1 : nk2;
+    const double* cf1 = coef1 + (size_t)s * ldc;
+    const double* cf2 = coef2 + (size_t)s * ldc;
+    double a1[kVecR], a2[kVecR];
+    long long idx[kVecR];
+    bool ok[kVecR];
+#pragma unroll
+    for (int r = 0; r < kVecR; ++r) {
+        idx[r] = r0 + threadIdx.x + (long long)r * kVecBS;
+        ok[r] = idx[r] < r1;
+        a1[r] = 0.0;
+        a2[r] = 0.0;
+    }
+    for (int k = 0; k < nk; ++k) {
+        const double* col = cols[k];
+        const bool u1 = k < nk1, u2 = k < nk2;
+        const double c1 = u1 ? cf1[k] : 0.0, c2 = u2 ? cf2[k] : 0.0;
+#pragma unroll
+        for (int r = 0; r < kVecR; ++r)
+            if (ok[r]) {
+                const double x = col[idx[r]];
+                if (u1) a1[r] += x * c1;
+                if (u2) a2[r] += x * c2;
+            }
+    }
+#pragma unroll
+    for (int r = 0; r < kVecR; ++r) {
+        if (!ok[r]) continue;
+        double o = base1[idx[r]];
+        o += a1[r];
+        out1[idx[r]] = o;
+        out2[idx[r]] = a2[r];
+    }
+}
+
 __global__ __launch_bounds__(kVecBS) void k_masked_copy(double* dst, const double* src, BlkMap M, const int* mask)
 {
     const int b = blockIdx.x;
@@ -672,6 +721,15 @@ hipError_t launch_lincomb(double* out, const double* base, const double* const* 
 {
     hipLaunchKernelGGL(k_lincomb, dim3(nblk), dim3(kVecBS), 0, st, out, base, cols, nk_s, coef, ldc, sgn, M, part, ldp,
                        partmode, u, at, rt, extra_sub, active);
+    return hipGetLastError();
+}
+
+hipError_t launch_lincomb2(double* out1, const double* base1, double* out2, const double* const* cols,
+                           const int* nk1_s, const double* coef1, const int* nk2_s, const double* coef2, int ldc,
+                           const BlkMap& M, int nblk, const int* active, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_lincomb2, dim3(nblk), dim3(kVecBS), 0, st, out1, base1, out2, cols, nk1_s, coef1, nk2_s, coef2,
+                       ldc, M, active);
     return hipGetLastError();
 }
 

@@ -101,10 +101,20 @@ struct tsr_solver {
     KrylovState KV{}, KW{};
     int* mask = nullptr;      // device activity mask
     int* nk = nullptr;        // per-scenario column counts for k_lincomb
+    // the step's second combination (the Arnoldi-relation term of D), formed in the same
+    // pass over the basis as U2: its coefficients and counts, the vector, and the
+    // one-column list {p2} with unit coefficients that turns it into D
+    double* coef2 = nullptr;
+    int* nk2 = nullptr;
+    double* p2 = nullptr;
+    const double** p2col = nullptr;
+    double* one_coef = nullptr;
+    int* one_nk = nullptr;
     int* bad = nullptr;       // alias of DBatch::bad
     std::vector<int> mask_h, nk_h, bad_h;
     std::vector<int> mask_dev_h;   // what the device mask holds (uploads only on change)
-    std::vector<double> coef_h, res_h, veps_h, cs0_h;
+    std::vector<double> coef_h, res_h, veps_h, cs0_h, coef2_h;
+    std::vector<int> nk2_h;
     std::vector<ScState> sc;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_g_ms = 0.0;
@@ -518,19 +528,13 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
             for (int k = 0; k < c.m; ++k) { c.y[k] = c.tau * c.beta * c.phi[k]; cf[k] = c.cs[k] * c.y[k]; }
             s->nk_h[q] = c.m;
         }
-        s->mask_h = pend;
-        if (upload_mask(s)) return -1;
-        HIPCHK(hipMemcpyAsync(s->KV.coef, s->coef_h.data(), sizeof(double) * S * s->KV.ld, hipMemcpyHostToDevice, s->st));
-        HIPCHK(hipMemcpyAsync(s->nk, s->nk_h.data(), sizeof(int) * S, hipMemcpyHostToDevice, s->st));
-        HIPCHK(launch_lincomb(s->u2, s->u, s->Vd, s->nk, s->KV.coef, s->KV.ld, 1.0, s->M, s->nblk, s->part, s->ldp, 0,
-                              s->u, o.AbsTol, o.RelTol, nullptr, s->mask, s->st));
-        if (G_at(s, s->u2, s->g2, s->mask)) return -1;   // bad flags cleared by the last readback
-        // D = g2 - g0 - V_{m+1} Hbar y
+        // the coefficients of V_{m+1} Hbar y (for D = g2 - g0 - V_{m+1} Hbar y): known now,
+        // so that combination is formed in the same pass over the basis as u2
         for (int q = 0; q < S; ++q) {
             ScState& c = s->sc[q];
-            double* cf = &s->coef_h[(size_t)q * s->KV.ld];
+            double* cf = &s->coef2_h[(size_t)q * s->KV.ld];
             std::fill(cf, cf + s->KV.ld, 0.0);
-            s->nk_h[q] = 0;
+            s->nk2_h[q] = 0;
             if (!pend[q] || !(c.beta > 0.0)) continue;
             const int mv = c.bd ? c.m : c.m + 1;
             for (int i = 0; i < mv; ++i) {
@@ -538,11 +542,20 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
                 for (int k = 0; k < c.m; ++k) acc += c.H[i + (size_t)k * ldh] * c.y[k];
                 cf[i] = c.cs[i] * acc;
             }
-            s->nk_h[q] = mv;
+            s->nk2_h[q] = mv;
         }
+        s->mask_h = pend;
+        if (upload_mask(s)) return -1;
         HIPCHK(hipMemcpyAsync(s->KV.coef, s->coef_h.data(), sizeof(double) * S * s->KV.ld, hipMemcpyHostToDevice, s->st));
         HIPCHK(hipMemcpyAsync(s->nk, s->nk_h.data(), sizeof(int) * S, hipMemcpyHostToDevice, s->st));
-        HIPCHK(launch_lincomb(s->D, s->g2, s->Vd, s->nk, s->KV.coef, s->KV.ld, -1.0, s->M, s->nblk, s->part, s->ldp, 1,
+        HIPCHK(hipMemcpyAsync(s->coef2, s->coef2_h.data(), sizeof(double) * S * s->KV.ld, hipMemcpyHostToDevice, s->st));
+        HIPCHK(hipMemcpyAsync(s->nk2, s->nk2_h.data(), sizeof(int) * S, hipMemcpyHostToDevice, s->st));
+        // u2 = u + V y and p2 = V_{m+1} Hbar y
+        HIPCHK(launch_lincomb2(s->u2, s->u, s->p2, s->Vd, s->nk, s->KV.coef, s->nk2, s->coef2, s->KV.ld, s->M, s->nblk,
+                               s->mask, s->st));
+        if (G_at(s, s->u2, s->g2, s->mask)) return -1;   // bad flags cleared by the last readback
+        // D = g2 - p2 - g0 (the one-column combination {p2} with coefficient 1)
+        HIPCHK(launch_lincomb(s->D, s->g2, s->p2col, s->one_nk, s->one_coef, 1, -1.0, s->M, s->nblk, s->part, s->ldp, 1,
                               s->u, o.AbsTol, o.RelTol, s->g0, s->mask, s->st));
         HIPCHK(launch_reduce(RED_SUM, s->part, s->ldp, 1, s->blk_first, S, s->scen_rows, s->KW, 0, s->mask, s->st));
         if (read_iteration(s, s->KW)) return -1;
@@ -863,9 +876,27 @@ tsr_solver* create(const tsr_batch_view* view, const exprem_options_struct* opt)
     hipMemset(s->KV.cs, 0, sizeof(double) * S * s->KV.ld);
     hipMemset(s->KW.cs, 0, sizeof(double) * S * s->KW.ld);
     if (dalloc(&s->mask, S) || dalloc(&s->nk, S)) return nullptr;
+    if (dalloc(&s->coef2, (size_t)S * s->KV.ld) || dalloc(&s->nk2, S) || dalloc(&s->p2, N) || dalloc(&s->p2col, 1) ||
+        dalloc(&s->one_coef, S) || dalloc(&s->one_nk, S)) {
+        set_error("hipMalloc failed for the step combinations");
+        return nullptr;
+    }
+    {
+        const std::vector<double> ones(S, 1.0);
+        const std::vector<int> ones_i(S, 1);
+        const double* pc = s->p2;
+        if (hipMemcpy(s->p2col, &pc, sizeof(pc), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(s->one_coef, ones.data(), sizeof(double) * S, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(s->one_nk, ones_i.data(), sizeof(int) * S, hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("hipMemcpy failed for the step combinations");
+            return nullptr;
+        }
+    }
     s->bad = batch_of(s->P).bad;
     s->mask_h.assign(S, 1); s->nk_h.assign(S, 0); s->bad_h.assign(S, 0);
     s->coef_h.assign((size_t)S * s->KV.ld, 0.0);
+    s->coef2_h.assign((size_t)S * s->KV.ld, 0.0);
+    s->nk2_h.assign(S, 0);
     s->res_h.assign((size_t)S * 8, 0.0);
     s->sc.resize(S);
     const int ldh = s->mcap + 1;
@@ -924,6 +955,7 @@ void tsr_solver_free(tsr_solver* s)
     }
     hipFree(s->KV.eps); hipFree(s->KV.ieps);
     hipFree(s->mask); hipFree(s->nk);
+    hipFree(s->coef2); hipFree(s->nk2); hipFree(s->p2); hipFree((void*)s->p2col); hipFree(s->one_coef); hipFree(s->one_nk);
     if (s->mb_h) hipHostFree(s->mb_h);
     if (s->linrec) hipFree(s->linrec);
     if (s->mstats) hipFree(s->mstats);
