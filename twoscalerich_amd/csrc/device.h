@@ -16,7 +16,7 @@ namespace tsr {
 constexpr int kMicroMaxBS = 1024;     // micro-cell workgroup size limit
 constexpr int kMaxNodeInc = 12;       // macro elements around a node hosting a cell
 constexpr int kVecBS = 256;           // threads per workgroup of the vector kernels
-constexpr int kVecR = 4;              // rows per thread of the vector kernels
+constexpr int kVecR = 2;              // rows per thread of the vector kernels
 constexpr int kMaxCols = 128;         // Krylov basis columns a pass can read
 
 struct DSoil { double Ks, tr, ts, al, n, m; };
