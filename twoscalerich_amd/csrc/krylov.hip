@@ -514,8 +514,16 @@ __global__ __launch_bounds__(256) void k_reduce(int mode, const double* part, in
     if (k >= nsum) return;
     const int b0 = blk_first[s], b1 = blk_first[s + 1];
     const double* col = part + (size_t)k * ldp;
-    double x = 0.0;
-    for (int b = b0 + threadIdx.x; b < b1; b += 256) x += col[b];
+    // thread t takes partials t, t + 256, ...: eight running sums (partials 8i + r go to
+    // sum r), so eight loads are in flight per thread, combined in a fixed tree
+    double xs[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int b = b0 + threadIdx.x;
+    for (; b + 7 * 256 < b1; b += 8 * 256) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) xs[r] += col[b + r * 256];
+    }
+    for (int r = 0; b < b1; b += 256, ++r) xs[r] += col[b];
+    double x = ((xs[0] + xs[1]) + (xs[2] + xs[3])) + ((xs[4] + xs[5]) + (xs[6] + xs[7]));
     x = wave_sum_v(x);
     const int tid = threadIdx.x;
     if ((tid & 63) == 0) red[tid >> 6] = x;
