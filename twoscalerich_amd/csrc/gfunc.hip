@@ -1204,7 +1204,7 @@ static std::vector<const void*> micro_kernels()
     std::vector<const void*> v;
     for (int st = 0; st < 2; ++st)
         for (int mode = 0; mode < 3; ++mode)
-            for (int w : {1, 6, 8}) v.push_back((const void*)micro_fn(st == 1, mode, w));
+            for (int w : kWpe) v.push_back((const void*)micro_fn(st == 1, mode, w));
     return v;
 }
 
@@ -1228,8 +1228,8 @@ static hipError_t tune_micro(DevProblem* p, const GArgs& a, hipStream_t st)
     const int mode = micro_mode(a);
     std::vector<int> cand_bs;
     for (int bs : {64, 128, 192, 256, 384, 512, 768, 1024}) {
-        // at most 8 strided nodes per thread (kLinNPT in the linearised mode)
-        // (the linearised mode: bs divides 256, at most kLinNPT nodes per thread)
+        // at most 8 strided nodes per thread; the linearised mode also needs bs to divide
+        // 256 and at most kLinNPT nodes per thread
         const bool ok2 = mode != 2 || (256 % bs == 0 && bs * kLinNPT >= B.nn_max);
         if (bs <= kMicroMaxBS && bs * 8 >= B.nn_max && ok2) cand_bs.push_back(bs);
     }
