@@ -71,16 +71,21 @@ def test_gfunc_reference_meshes(model):
 
 
 def test_gfunc_hip_reference_signature():
-    """Gfunc_hip(g, u, user_data) with the reference's by-value struct."""
-    p = T.Problem.config(F.CFG_PLAIN_SMALL)
-    u = seeded_state(p, 3)
-    g = np.empty_like(u)
+    """Gfunc_hip(g, u, user_data) with the reference's by-value struct.  The solver it
+    makes for a problem is freed with the problem: the second problem (the first one's
+    memory may be reused) gets a fresh one."""
     L = T.lib()
-    rc = L.Gfunc_hip(T._dp(g), T._dp(u), p.ud.contents)
-    o = Oracle(p.view)
-    g_ref, _ = o.gfunc(u)
-    assert rc == 0 and rel_inf(g, g_ref) <= G_TOL
-    o.close()
+    for mn in (0, 7):
+        p = T.Problem.config(F.CFG_PLAIN_SMALL, mn)
+        u = seeded_state(p, 3)
+        g = np.empty_like(u)
+        rc = L.Gfunc_hip(T._dp(g), T._dp(u), p.ud.contents)
+        o = Oracle(p.view)
+        g_ref, _ = o.gfunc(u)
+        print(f"Gfunc_hip N={p.N}: rc {rc} rel_inf {rel_inf(g, g_ref):.3e}", flush=True)
+        assert rc == 0 and rel_inf(g, g_ref) <= G_TOL
+        o.close()
+        p.close()
 
 
 def test_exprem_steps_match_oracle():
@@ -128,3 +133,35 @@ def test_exprem_c_reference_signature():
                         T.C.byref(st))
         assert rc == 0
     assert st.NumSteps == 2 and st.FuncEvals > 2 and 0 < t.value < 0.05
+
+
+def test_exprem_c_host_gfunc_matches_oracle():
+    """exprem_c with a host RHS (any function other than Gfunc_hip): G runs on the host at
+    every evaluation, the Krylov arithmetic on the device.  With the oracle's own
+    Gfunc_parallel as that function, 10 steps follow the oracle's EXPREM step for step."""
+    p = T.Problem.config(F.CFG_PLAIN_SMALL)
+    L = T.lib()
+    o = Oracle(p.view, p.ud)
+    opt = p.options()
+    tend = p.tend()
+    gf = T.C.cast(o.lib.Gfunc_parallel, T.C.c_void_p)
+    u = p.initial()
+    u_cpu = u.copy()
+    t = T.C.c_double(0.0)
+    st = F.exprem_stats_struct()
+    st_cpu = F.exprem_stats_struct()
+    t_cpu = 0.0
+    for k in range(10):
+        rc = L.exprem_c(gf, p.N, T.C.byref(t), tend, None, None, T._dp(u), p.ud.contents, opt, T.C.byref(st))
+        assert rc == 0
+        rc2, t_cpu, last = o.exprem_step(p.ud.contents, u_cpu, t_cpu, tend, opt, st_cpu)
+        assert rc2 == 0
+        print(f"step {k}: host-G t={t.value:.6e} evals={st.FuncEvals} | oracle t={t_cpu:.6e} "
+              f"evals={st_cpu.FuncEvals}", flush=True)
+    assert st.NumSteps == st_cpu.NumSteps == 10
+    assert abs(t.value - t_cpu) <= 1e-8 * t_cpu
+    err = rel_inf(u, u_cpu)
+    print(f"after 10 steps: rel_inf(u)={err:.3e}", flush=True)
+    assert err <= 1e-6
+    o.close()
+    p.close()

@@ -33,6 +33,7 @@ struct tsr_problem {
     ~tsr_problem()
     {
         tsr::registry_drop(&view);
+        tsr::solver_cache_drop(&view);
         for (auto& m : macros) tsr::free_macro(m);
         for (auto& t : tmpl) tsr::free_micro(t);
         for (auto& k : keffs) { delete[] k.logh_values; delete[] k.Keff_values; }

@@ -159,7 +159,9 @@ namespace {
 
 std::mutex g_cache_mu;
 std::map<const tsr_batch_view*, tsr_solver*> g_solver_cache;
-std::vector<tsr_solver*> g_retired;   // solvers replaced when a caller changed options; freed at tsr_solver_free of the cache
+// solvers replaced when a caller changed options, with their problem's view; freed
+// with the problem (solver_cache_drop)
+std::vector<std::pair<const tsr_batch_view*, tsr_solver*>> g_retired;
 
 template <class T> int dalloc(T** p, size_t n)
 {
@@ -928,6 +930,32 @@ tsr_solver* create(const tsr_batch_view* view, const exprem_options_struct* opt)
 
 }  // namespace
 
+namespace tsr {
+// A problem is being freed: free the solvers the reference-signature entry points made
+// for it, so that a later problem allocated at the same address starts afresh.
+void solver_cache_drop(const tsr_batch_view* v)
+{
+    std::vector<tsr_solver*> drop;
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        auto it = g_solver_cache.find(v);
+        if (it != g_solver_cache.end()) {
+            drop.push_back(it->second);
+            g_solver_cache.erase(it);
+        }
+        for (auto r = g_retired.begin(); r != g_retired.end();) {
+            if (r->first == v) {
+                drop.push_back(r->second);
+                r = g_retired.erase(r);
+            } else {
+                ++r;
+            }
+        }
+    }
+    for (tsr_solver* s : drop) tsr_solver_free(s);
+}
+}  // namespace tsr
+
 extern "C" {
 
 tsr_solver* tsr_solver_create(const tsr_batch_view* view, const exprem_options_struct* opt)
@@ -1229,7 +1257,7 @@ static tsr_solver* solver_for(const user_data_struct& ud, const exprem_options_s
     if (it != g_solver_cache.end()) {
         // a caller changed options: keep the old solver alive (its device state may
         // still be referenced by that caller's loop) and cache the new one
-        g_retired.push_back(it->second);
+        g_retired.push_back({v, it->second});
         g_solver_cache.erase(it);
     }
     g_solver_cache[v] = s;
