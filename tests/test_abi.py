@@ -58,3 +58,17 @@ def test_host_dense_routines_and_helpers():
     assert (o.AbsTol, o.RelTol, o.MinKrylov, o.MaxKrylov, o.MinStep, o.MaxStep, o.MaxNumSteps, o.SafetyFac,
             o.MinStepDecFac, o.MaxStepIncFac, o.InitStep) == (1e-5, 1e-5, 2, 100, 1e-10, 1000.0, 100000, 0.8, 0.1,
                                                             1.1, 1e-3)
+
+
+def test_solver_create_rejects_bad_options_without_a_gpu():
+    # checked before any device work: NULL and a message; the partly built solver is
+    # released
+    import pytest
+    p = T.Problem.config(F.CFG_PLAIN_SMALL)
+    o = p.options()
+    o.MaxKrylov = 0
+    assert T.lib().tsr_solver_create(p.view, C.byref(o)) is None
+    assert b"MaxKrylov" in T.lib().tsr_last_error()
+    with pytest.raises(RuntimeError, match="MaxKrylov"):
+        T.Solver(p, o)
+    p.close()

@@ -803,11 +803,16 @@ void prefer_spin_sync()
     std::call_once(once, [] { (void)hipSetDeviceFlags(hipDeviceScheduleSpin); (void)hipGetLastError(); });
 }
 
+// a solver whose set-up fails part way is released like a finished one
+struct SolverRelease {
+    void operator()(tsr_solver* s) const { tsr_solver_free(s); }
+};
+
 tsr_solver* create(const tsr_batch_view* view, const exprem_options_struct* opt)
 {
     prefer_spin_sync();
     std::string err;
-    std::unique_ptr<tsr_solver> s(new tsr_solver());
+    std::unique_ptr<tsr_solver, SolverRelease> s(new tsr_solver());
     s->opt = *opt;
     if (opt->MaxKrylov < 1 || opt->MaxKrylov > kMaxKrylovCap) { set_error("MaxKrylov must be in [1, 120]"); return nullptr; }
     s->mcap = kMaxKrylovCap;
@@ -853,13 +858,13 @@ tsr_solver* create(const tsr_batch_view* view, const exprem_options_struct* opt)
     int *d_bs = nullptr; long long *d_b0 = nullptr, *d_b1 = nullptr;
     rc = dalloc(&d_bs, bs.size()) | dalloc(&d_b0, b0.size()) | dalloc(&d_b1, b1.size()) | dalloc(&s->blk_first, S + 1) |
          dalloc(&s->scen_rows, S);
+    s->M.scen = d_bs; s->M.r0 = d_b0; s->M.r1 = d_b1;
     if (rc) return nullptr;
     hipMemcpy(d_bs, bs.data(), sizeof(int) * bs.size(), hipMemcpyHostToDevice);
     hipMemcpy(d_b0, b0.data(), sizeof(long long) * b0.size(), hipMemcpyHostToDevice);
     hipMemcpy(d_b1, b1.data(), sizeof(long long) * b1.size(), hipMemcpyHostToDevice);
     hipMemcpy(s->blk_first, bf.data(), sizeof(int) * (S + 1), hipMemcpyHostToDevice);
     hipMemcpy(s->scen_rows, s->rows_h.data(), sizeof(long long) * S, hipMemcpyHostToDevice);
-    s->M.scen = d_bs; s->M.r0 = d_b0; s->M.r1 = d_b1;
     s->ldp = s->nblk;   // partials are column-major: part[k * nblk + b], k < mcap + 2
     if (dalloc(&s->part, (size_t)s->nblk * (2 * s->mcap + 4))) return nullptr;   // DCGS2 needs 2 (j+1) + 1 columns
     auto kstate = [&](KrylovState& K, double* shared_eps, double* shared_ieps) {
@@ -971,7 +976,7 @@ void tsr_solver_free(tsr_solver* s)
         for (auto it = g_solver_cache.begin(); it != g_solver_cache.end();)
             it = (it->second == s) ? g_solver_cache.erase(it) : std::next(it);
     }
-    hipDeviceSynchronize();
+    if (s->P) hipDeviceSynchronize();   // no kernel ran before the problem was uploaded
     for (double* p : {s->u, s->g0, s->u2, s->g2, s->D, s->unew, s->scratch[0], s->scratch[1]}) if (p) hipFree(p);
     for (size_t k = 1; k < s->V.size(); ++k) { if (s->V[k]) hipFree(s->V[k]); if (s->W[k]) hipFree(s->W[k]); }
     hipFree(s->Vd); hipFree(s->Wd);
