@@ -35,4 +35,27 @@ int oracle_exprem(int (*Gfunc)(double*, double*, user_data_struct), int N, doubl
 void oracle_exprem_last(double* out8);
 }
 
+#include <vector>
+
+// sum_{i < n} f(i) in an order fixed by n alone: 4096-term chunks summed in parallel, the
+// chunk sums added in chunk order, so the result does not depend on the thread count or
+// on which thread finishes first (an OpenMP reduction clause combines in arrival order).
+template <class F>
+inline double oracle_fixed_sum(long long n, F f)
+{
+    const long long kChunk = 4096;
+    const long long nc = (n + kChunk - 1) / kChunk;
+    std::vector<double> part((size_t)(nc > 0 ? nc : 1), 0.0);
+#pragma omp parallel for schedule(static)
+    for (long long c = 0; c < nc; ++c) {
+        double s = 0.0;
+        const long long e = (c + 1) * kChunk < n ? (c + 1) * kChunk : n;
+        for (long long i = c * kChunk; i < e; ++i) s += f(i);
+        part[(size_t)c] = s;
+    }
+    double s = 0.0;
+    for (long long c = 0; c < nc; ++c) s += part[(size_t)c];
+    return s;
+}
+
 #endif

@@ -172,12 +172,10 @@ int oracle_phik_e1(double* out, const double* H, int ldh, int m, double tau, int
 // sqrt( (1/N) sum (e_i / (AbsTol + RelTol |u_i|))^2 )   (include/exprem.h:27)
 double oracle_weighted_norm(const double* e, const double* u, double AbsTol, double RelTol, long long N)
 {
-    double s = 0.0;
-#pragma omp parallel for reduction(+ : s) schedule(static)
-    for (long long i = 0; i < N; ++i) {
+    const double s = oracle_fixed_sum(N, [&](long long i) {
         const double r = e[i] / (AbsTol + RelTol * std::fabs(u[i]));
-        s += r * r;
-    }
+        return r * r;
+    });
     return std::sqrt(s / (double)N);
 }
 

@@ -35,10 +35,7 @@ constexpr double kKrylovTol = 0.1;
 
 double dot(const double* a, const double* b, long long n)
 {
-    double s = 0.0;
-#pragma omp parallel for reduction(+ : s) schedule(static)
-    for (long long i = 0; i < n; ++i) s += a[i] * b[i];
-    return s;
+    return oracle_fixed_sum(n, [&](long long i) { return a[i] * b[i]; });
 }
 
 double wrms(const double* e, const double* u, double at, double rt, long long n)

@@ -202,12 +202,12 @@ inline void macro_alpha(const tsr_scenario& sc, const Macro& m, const double* u,
     const int ne = m.ne, nv = m.nv;
     double hv[4];
     for (int i = 0; i < nv; ++i) hv[i] = macro_head(sc, m, u, M.elements[i * ne + e]);
+    al[0] = al[1] = al[2] = al[3] = 0.0;   // triangles use the first three
     for (int k = 0; k < nv; ++k) {
         double a = 0.0;
         for (int i = 0; i < nv; ++i) a += hv[i] * M.shape_funcs[(k * nv + i) * ne + e];
         al[k] = a;
     }
-    if (nv == 3) al[3] = 0.0;
 }
 
 inline double interp_h(int nv, const double al[4], double x, double y)

@@ -72,3 +72,18 @@ def test_solver_create_rejects_bad_options_without_a_gpu():
     with pytest.raises(RuntimeError, match="MaxKrylov"):
         T.Solver(p, o)
     p.close()
+
+
+def test_gfunc_hip_rejects_unregistered_user_data_of_unknown_shape():
+    # a user_data_struct no set-up call registered is taken as the reference's own problem
+    # only when N fits one of its two hosting rules; otherwise Gfunc_hip returns 2 with the
+    # reason, before any device work
+    import numpy as np
+    p = T.Problem.config(F.CFG_PLAIN_SMALL)
+    ud = F.user_data_struct.from_buffer_copy(p.ud.contents)
+    ud.N += 1
+    u = np.zeros(ud.N)
+    g = np.zeros(ud.N)
+    assert T.lib().Gfunc_hip(T._dp(g), T._dp(u), ud) == 2
+    assert b"M_nodes + M_elements*m_nodes" in T.lib().tsr_last_error()
+    p.close()

@@ -88,6 +88,32 @@ def test_gfunc_hip_reference_signature():
         p.close()
 
 
+def test_gfunc_hip_unregistered_user_data():
+    """The reference driver's own structs, which no set-up call registered (here: a copy
+    of a registered problem's user_data_struct with its own element array): Gfunc_hip
+    builds the reference's problem around them (cells hosted by macro elements, soil B
+    in every micro element), the problem tsr_problem_reference also builds."""
+    import ctypes as C
+    p = reference_problem(model=3)
+    ud = F.user_data_struct.from_buffer_copy(p.ud.contents)
+    M = ud.macro_mesh
+    el = np.ctypeslib.as_array(M.elements, shape=(M.no_verts * M.no_elements,)).copy()
+    ud.macro_mesh.elements = el.ctypes.data_as(C.POINTER(C.c_int))
+    u = seeded_state(p, 7, -3.0, -0.2)
+    L = T.lib()
+    g1 = np.empty_like(u)
+    g2 = np.empty_like(u)
+    assert L.Gfunc_hip(T._dp(g1), T._dp(u), ud) == 0, L.tsr_last_error()
+    assert L.Gfunc_hip(T._dp(g2), T._dp(u), ud) == 0   # the same problem, found again
+    o = Oracle(p.view)
+    g_ref, _ = o.gfunc(u)
+    err = rel_inf(g1, g_ref)
+    print(f"unregistered user_data: N={p.N} rel_inf={err:.3e}", flush=True)
+    assert err <= G_TOL and np.array_equal(g1, g2)
+    o.close()
+    p.close()
+
+
 def test_exprem_steps_match_oracle():
     """20 EXPREM steps of the plain small workload on the GPU and on the CPU oracle."""
     p = T.Problem.config(F.CFG_PLAIN_SMALL)

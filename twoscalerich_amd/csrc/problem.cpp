@@ -29,13 +29,15 @@ struct tsr_problem {
     double t_end = 1.0;
     exprem_options_struct opt{};
     long long distinct_bytes = 0;
+    bool borrowed = false;                   // tmpl holds the caller's micro mesh (not owned)
 
     ~tsr_problem()
     {
         tsr::registry_drop(&view);
         tsr::solver_cache_drop(&view);
         for (auto& m : macros) tsr::free_macro(m);
-        for (auto& t : tmpl) tsr::free_micro(t);
+        if (!borrowed)
+            for (auto& t : tmpl) tsr::free_micro(t);
         for (auto& k : keffs) { delete[] k.logh_values; delete[] k.Keff_values; }
     }
 };
@@ -229,6 +231,80 @@ void registry_drop(const tsr_batch_view* v)
     std::lock_guard<std::mutex> lk(g_mu);
     for (auto it = g_map.begin(); it != g_map.end();)
         it = (it->second == v) ? g_map.erase(it) : std::next(it);
+}
+
+namespace {
+std::mutex g_adopt_mu;
+std::vector<tsr_problem*> g_adopted;   // kept for the life of the process
+
+// the adopted problem still describes the caller's structs (same arrays, soils, flux)
+bool same_problem(const user_data_struct& a, const user_data_struct& b)
+{
+    const macro_mesh_struct &Ma = a.macro_mesh, &Mb = b.macro_mesh;
+    const micro_mesh_struct &ma = a.micro_mesh, &mb = b.micro_mesh;
+    auto same_soil = [](const soil_struct& x, const soil_struct& y) {
+        return std::memcmp(&x, &y, sizeof(soil_struct)) == 0;
+    };
+    return Ma.no_verts == Mb.no_verts && Ma.no_nodes == Mb.no_nodes && Ma.no_elements == Mb.no_elements &&
+           Ma.nodes == Mb.nodes && Ma.boundary_condition_edges == Mb.boundary_condition_edges &&
+           ma.no_nodes == mb.no_nodes && ma.no_elements == mb.no_elements && ma.elements == mb.elements &&
+           ma.nodes == mb.nodes && a.effective_conductivity.Keff_values == b.effective_conductivity.Keff_values &&
+           same_soil(a.soilA, b.soilA) && same_soil(a.soilB, b.soilB) && a.boundary_flux == b.boundary_flux &&
+           a.model_type == b.model_type;
+}
+}  // namespace
+
+// The view for a user_data_struct: the one a set-up call registered for (elements, N),
+// else the reference's own problem built around the caller's structs (one micro mesh,
+// soil B in every micro element; cells hosted by macro elements when N = M_nodes +
+// M_elements*m_nodes, main.cpp:260, by macro nodes when N = M_nodes*(1 + m_nodes)).
+// The latter borrows the caller's mesh, Keff and soil data; it is uploaded when a solver
+// is made for it, so the arrays need only live until then.
+const tsr_batch_view* registry_find(const user_data_struct& ud, std::string& err)
+{
+    std::lock_guard<std::mutex> lk(g_adopt_mu);
+    const tsr_batch_view* v = registry_get(ud);
+    tsr_problem* prev = nullptr;
+    for (tsr_problem* q : g_adopted)
+        if (v == &q->view) prev = q;
+    if (v && (!prev || same_problem(prev->scen[0].ud, ud))) return v;
+    const macro_mesh_struct& M = ud.macro_mesh;
+    const micro_mesh_struct& m = ud.micro_mesh;
+    if (!M.elements || !m.elements || M.no_nodes <= 0 || M.no_elements <= 0 || m.no_nodes <= 0) {
+        err = "user_data_struct has no macro or micro mesh";
+        return nullptr;
+    }
+    const long long N = ud.N, Mn = M.no_nodes, mn = m.no_nodes;
+    int host;
+    if (N == Mn + (long long)M.no_elements * mn) {
+        host = TSR_HOST_ELEMENT;
+    } else if (N == Mn * (1 + mn)) {
+        host = TSR_HOST_NODE;
+    } else {
+        err = "user_data.N = " + std::to_string(N) + " is neither M_nodes + M_elements*m_nodes nor " +
+              "M_nodes*(1 + m_nodes), and no set-up call registered this problem";
+        return nullptr;
+    }
+    if (m.no_verts != 3) {
+        err = "micro mesh must be triangles";
+        return nullptr;
+    }
+    std::unique_ptr<tsr_problem> p(new tsr_problem());
+    p->borrowed = true;
+    p->tmpl.push_back(m);
+    p->soil_store.emplace_back();   // soil B in every micro element
+    tsr_scenario sc{};
+    sc.ud = ud;
+    sc.host_kind = host;
+    sc.n_cells = host == TSR_HOST_NODE ? M.no_nodes : M.no_elements;
+    sc.dirichlet_head = ud.boundary_flux;
+    p->scen.push_back(sc);
+    p->cell_tmpl.assign(sc.n_cells, 0);
+    p->cell_offset.assign(sc.n_cells, 0);
+    set_exprem_options(p->opt);
+    finish(p.get(), 0.0);   // registers (elements, N)
+    g_adopted.push_back(p.release());
+    return &g_adopted.back()->view;
 }
 
 void set_error(const std::string& s) { g_err = s; }

@@ -1248,8 +1248,9 @@ int tsr_d2h(double* dst, const double* src, long long n)
 // ---------------------------------------------------------------- reference signatures
 static tsr_solver* solver_for(const user_data_struct& ud, const exprem_options_struct* opt)
 {
-    const tsr_batch_view* v = registry_get(ud);
-    if (!v) { set_error("no problem registered for this user_data (use tsr_problem_*)"); return nullptr; }
+    std::string err;
+    const tsr_batch_view* v = registry_find(ud, err);
+    if (!v) { set_error(err); return nullptr; }
     std::lock_guard<std::mutex> lk(g_cache_mu);
     auto it = g_solver_cache.find(v);
     if (it != g_solver_cache.end() && (!opt || std::memcmp(&it->second->opt, opt, sizeof(*opt)) == 0)) return it->second;

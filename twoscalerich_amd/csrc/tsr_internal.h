@@ -35,6 +35,9 @@ double vg_K_host(const soil_struct& s, double h);
 void registry_put(const user_data_struct& ud, const tsr_batch_view* v);
 const tsr_batch_view* registry_get(const user_data_struct& ud);
 void registry_drop(const tsr_batch_view* v);
+// registry_get, else the reference's own problem around the caller's structs (err set
+// when N fits neither hosting rule)
+const tsr_batch_view* registry_find(const user_data_struct& ud, std::string& err);
 // free the solvers that Gfunc_hip / exprem made for a problem's view (solver.cpp)
 void solver_cache_drop(const tsr_batch_view* v);
 
