@@ -32,7 +32,8 @@ Extra fields on the JSON line:
                 traffic = FETCH+WRITE bytes per launch from rocprofv3 counters
                 (profiles/), scaled by the copy-kernel calibration, or null; it exceeds
                 the algorithmic bytes because the linearised form also streams the
-                six recorded expansion components per micro node.
+                recorded expansion (three FP64 values and three FP32 derivatives per
+                interior micro node).
   cpu_baseline  the CPU oracle (FP64 C++/OpenMP restatement of the plain algorithm,
                 "port") timed on this host: EXPREM steps of the same workload for a
                 bounded sample, scaled to t_end by the plain GPU path's time profile.
@@ -179,7 +180,10 @@ def main():
     n_dir = n_cells * (4 * (int(round(r ** 0.5)) - 1))
     n_int = n_micro - n_dir
     lin_bytes = 32.0 * p.N + p.distinct_bytes()
-    lin_impl_bytes = 80.0 * n_int + 56.0 * n_dir + p.distinct_bytes()   # what the linearised form reads and writes
+    # what the linearised form reads and writes: interior node u, v, g0, 3 FP64 + 3 FP32 recorded
+    # components, the product; Dirichlet node g0, the recorded head, 2 FP64 + 2 FP32 conductivity
+    # components, the product
+    lin_impl_bytes = 68.0 * n_int + 48.0 * n_dir + p.distinct_bytes()
     lin_ms = gf["linearised_fd"]["micro_ms"]
     achieved = lin_bytes / (lin_ms * 1e-3) / 1e9
     full_bytes = 16.0 * p.N + p.distinct_bytes()

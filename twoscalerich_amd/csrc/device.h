@@ -115,12 +115,18 @@ struct DBatch {
 //   value written = (G - g0) * ieps[s]   (g0 may be null)
 //
 // Linearised Jacobian products: a G(u) launch with rec set also writes, per micro node
-// (SoA, component c at rec[c * N + global index]): 0 the head used, 1 the node's storage
-// capacity sum(scv C), 2 its derivative, 3..6 K and dK/dh of soils A and B (0 where the
-// soil is absent).  A launch with lin set (an FD product around that same u) takes the
-// micro nodes' capacities and conductivities from the first-order expansion
-// X(h) = X0 + X0' (h - h0) instead of the van Genuchten functions.
-constexpr int kLinComp = 7;
+// (SoA over the global index): FP64 components 0 the head used, 1 the node's storage
+// capacity sum(scv C), 2 K of soil A, 3 K of soil B (rec[c * N + i]); then, from
+// rec + 4 N viewed as floats, FP32 components 0 d(storage)/dh, 1 dK_A/dh, 2 dK_B/dh
+// ((float*)(rec + 4 N))[c * N + i]; K and dK/dh are 0 where the soil is absent.  A launch
+// with lin set (an FD product around that same u) takes the micro nodes' capacities and
+// conductivities from the first-order expansion X(h) = X0 + X0' (h - h0) instead of the
+// van Genuchten functions.  X0' multiplies h - h0 = eps v, so its FP32 rounding (2^-24
+// relative) changes the product by that factor of a term of order eps; X0 itself, which
+// the product differences against G(u), stays FP64.
+constexpr int kLinComp64 = 4;
+constexpr int kLinComp32 = 3;
+constexpr size_t kLinBytesPerNode = 8 * kLinComp64 + 4 * kLinComp32;
 struct GArgs {
     const double* u;
     const double* v;

@@ -456,9 +456,10 @@ __device__ __forceinline__ double generic_net(const DTmpl& T, int i, const doubl
 constexpr int kLinNPT = 4;
 
 // What a linearised FD product reads of one micro node: g0 at every node; at interior
-// nodes the state, the direction and recorded components 1..6 (storage, its derivative,
-// K_A, K_A', K_B, K_B').  Loading into this register block first and applying later lets
-// a thread issue every load of its nodes before it waits on any of them.
+// nodes the state, the direction and the recorded expansion c = (storage, its derivative,
+// K_A, K_A', K_B, K_B'), values from the FP64 components and derivatives from the FP32
+// ones.  Loading into this register block first and applying later lets a thread issue
+// every load of its nodes before it waits on any of them.
 struct LinNode {
     double g0, u, v, c[6];
 };
@@ -474,8 +475,12 @@ __device__ __forceinline__ void lin_load(const DTmpl& T, const GArgs& a, size_t 
     x.u = a.u[gi];
     x.v = a.v ? a.v[gi] : 0.0;
     const double* L = a.lin + gi;
+    const float* Ld = reinterpret_cast<const float*>(a.lin + kLinComp64 * NN) + gi;
 #pragma unroll
-    for (int c = 0; c < 6; ++c) x.c[c] = L[(c + 1) * NN];
+    for (int c = 0; c < 3; ++c) {
+        x.c[2 * c] = L[(c + 1) * NN];
+        x.c[2 * c + 1] = (double)Ld[c * NN];
+    }
 }
 
 // Recorded head and the four conductivity components at a Dirichlet node.
@@ -486,9 +491,13 @@ struct LinDir {
 __device__ __forceinline__ void lin_load_dir(const GArgs& a, size_t NN, long long gi, LinDir& x)
 {
     const double* L = a.lin + gi;
+    const float* Ld = reinterpret_cast<const float*>(a.lin + kLinComp64 * NN) + gi;
     x.hb = L[0];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) x.c[c] = L[(c + 3) * NN];
+    for (int c = 0; c < 2; ++c) {
+        x.c[2 * c] = L[(c + 2) * NN];
+        x.c[2 * c + 1] = (double)Ld[(c + 1) * NN];
+    }
 }
 
 __device__ __forceinline__ void store_fd(const DBatch& B, const GArgs& a, int s, long long i, double g, double g0v)
@@ -699,11 +708,12 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
                     double* R = a.rec + off + i;
                     R[0] = h[i];
                     R[NN] = st;
-                    R[2 * NN] = dcA * cva + dcB * cvb;
-                    R[3 * NN] = kA;
-                    R[4 * NN] = dkA;
-                    R[5 * NN] = kB;
-                    R[6 * NN] = dkB;
+                    R[2 * NN] = kA;
+                    R[3 * NN] = kB;
+                    float* Rd = reinterpret_cast<float*>(a.rec + kLinComp64 * NN) + off + i;
+                    Rd[0] = (float)(dcA * cva + dcB * cvb);
+                    Rd[NN] = (float)dkA;
+                    Rd[2 * NN] = (float)dkB;
                 }
             }
         }

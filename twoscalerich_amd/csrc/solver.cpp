@@ -133,7 +133,7 @@ struct tsr_solver {
     double* mb_d = nullptr;     // ... and its device address
     int mb_stride = 0;          // doubles per scenario
     int path = TSR_PATH_PLAIN;
-    double* linrec = nullptr;   // TSR_PATH_JVLIN: expansion point of the FD products (kLinComp x N)
+    double* linrec = nullptr;   // TSR_PATH_JVLIN: expansion point of the FD products (kLinBytesPerNode x N bytes)
     double t_dense = 0.0;     // host seconds in the small dense phi routines
     double t_sync = 0.0;      // host seconds waiting for the stream
 };
@@ -1203,7 +1203,7 @@ int tsr_solver_set_path(tsr_solver* s, int path)
         return 1;
     }
     if ((path & TSR_PATH_JVLIN) && !s->linrec) {
-        if (hipMalloc((void**)&s->linrec, sizeof(double) * kLinComp * (size_t)s->N) != hipSuccess) {
+        if (hipMalloc((void**)&s->linrec, kLinBytesPerNode * (size_t)s->N) != hipSuccess) {
             s->linrec = nullptr;
             set_error("hipMalloc failed for the linearisation record");
             return 2;
