@@ -99,6 +99,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--chunk", type=int, default=1000, help="EXPREM steps between progress lines")
+    ap.add_argument("--basis", choices=("f32", "f64"), default="f32",
+                    help="storage of the Krylov basis columns on the batched path (arithmetic is FP64)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only time the G kernels in their three modes on a perturbed initial state (for profilers)")
     args = ap.parse_args()
@@ -120,7 +122,7 @@ def main():
     u0 = p.initial()
 
     fast = T.Solver(p, opt)
-    fast.set_path(F.PATH_BATCHED)
+    fast.set_path(F.PATH_BATCHED | (F.PATH_F32BASIS if args.basis == "f32" else 0))
     CT.attach(fast)
     if args.kernels_only:
         rng = np.random.default_rng(1)
@@ -233,7 +235,8 @@ def main():
                                "centred sand disc r=0.25, Dirichlet -0.1 m top, h0=-5 m, t in [0, 1] d",
                    "N": p.N, "t_end_days": tend, "rtol": opt.RelTol, "atol": opt.AbsTol,
                    "max_krylov_plain": opt.MaxKrylov, "controller": "learned (Krylov range [5, 60])",
-                   "path": "batched (DCGS2, deferred Krylov checks, linearised FD products)"},
+                   "path": "batched (DCGS2" + (" on FP32 copies of the basis" if args.basis == "f32" else "")
+                           + ", deferred Krylov checks, linearised FD products)"},
         "exprem": {"accepted_steps": st["steps"], "rejected": st["rejects"], "rhs_evals": st["fevals"],
                    "host_dense_s": round(st["host_dense_s"], 3), "sync_wait_s": round(st["sync_wait_s"], 3),
                    "u_end_min": float(u_end.min()), "u_end_max": float(u_end.max())},

@@ -231,9 +231,13 @@ int   tsr_mlp_train(void* mlp, const float* x, const float* t, int n, int epochs
  *                      expanded to first order about the step's u (the FD quotient then
  *                      differs from the full one by O(eps) terms, as the FD quotient
  *                      itself differs from J v)
- * Returns 0, 1 on unknown bits, 2 on allocation failure. */
+ *   TSR_PATH_F32BASIS  with DCGS2: the Gram-Schmidt passes and the step combinations read
+ *                      FP32 copies of the Krylov basis columns (FP64 arithmetic; the
+ *                      newest column and every FD product stay FP64), halving the
+ *                      basis traffic; the basis is then orthonormal to FP32 rounding
+ * Returns 0, 1 on unknown bits (or F32BASIS without DCGS2), 2 on allocation failure. */
 enum { TSR_PATH_PLAIN = 0, TSR_PATH_DCGS2 = 1, TSR_PATH_DEFERRED = 2, TSR_PATH_JVLIN = 4, TSR_PATH_MONOTONE = 8,
-       TSR_PATH_BATCHED = 15 };
+       TSR_PATH_BATCHED = 15, TSR_PATH_F32BASIS = 16 };
 int tsr_solver_set_path(tsr_solver* s, int path);
 /* jv = (G(u + eps v) - G(u)) / eps about the solver's state u, with the full RHS
  * (lin = 0) or the TSR_PATH_JVLIN expansion (lin = 1); v, jv host arrays of N. */

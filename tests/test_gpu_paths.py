@@ -5,8 +5,8 @@
   * the linearised FD product agrees with the full one to O(eps), reproduces G(u)
     exactly at a zero perturbation, and its linearisation error is second order in the
     perturbation (a wrong derivative would make it first order);
-  * integrations to t_end of the small workload by every path land within 1e-6 of the
-    oracle's solution.
+  * integrations to t_end of the small workload by every path (the FP32-basis one too)
+    land within 1e-6 of the oracle's solution.
 """
 import numpy as np
 import pytest
@@ -110,7 +110,8 @@ def oracle_end():
     return u, n, st.FuncEvals
 
 
-@pytest.mark.parametrize("path", [F.PATH_PLAIN, F.PATH_DCGS2 | F.PATH_DEFERRED, F.PATH_BATCHED])
+@pytest.mark.parametrize("path", [F.PATH_PLAIN, F.PATH_DCGS2 | F.PATH_DEFERRED, F.PATH_BATCHED,
+                                  F.PATH_BATCHED | F.PATH_F32BASIS])
 def test_path_to_tend_matches_oracle(oracle_end, path):
     u_ref, n_ref, fe_ref = oracle_end
     p, s = _solver(path=path)
@@ -121,3 +122,20 @@ def test_path_to_tend_matches_oracle(oracle_end, path):
     assert t == p.tend()
     assert err <= 1e-6
     s.close(); p.close()
+
+
+def test_f32_basis_needs_dcgs2_and_tracks_the_f64_basis():
+    """TSR_PATH_F32BASIS: rejected without DCGS2; with it, 40 steps stay within FP32
+    rounding of the same path on the FP64 basis."""
+    p, a = _solver(path=F.PATH_BATCHED)
+    assert T.lib().tsr_solver_set_path(a.h, F.PATH_F32BASIS | F.PATH_DEFERRED) == 1
+    ua, ta, ra, _ = _run(a, p.tend(), 40)
+    _, b = _solver(path=F.PATH_BATCHED | F.PATH_F32BASIS)
+    ub, tb, rb, _ = _run(b, p.tend(), 40)
+    ma = np.array([r.m for r in ra]); mb = np.array([r.m for r in rb])
+    d = rel_inf(ub, ua)
+    print(f"[paths] f32 basis: t {ta:.6e} vs {tb:.6e}, m differs in {np.sum(ma != mb)} of {len(ma)} steps, "
+          f"rel {d:.2e}", flush=True)
+    assert abs(ta - tb) <= 1e-6 * ta
+    assert d <= 1e-6
+    a.close(); b.close(); p.close()

@@ -89,7 +89,7 @@ GFUNC_T = C.CFUNCTYPE(C.c_int, _dp, _dp, user_data_struct)
 
 HOST_ELEMENT, HOST_NODE = 0, 1
 CFG_PLAIN_SMALL, CFG_PLAIN, CFG_RAGGED, CFG_SCENARIOS, CFG_LARGE = 0, 1, 2, 3, 4
-PATH_PLAIN, PATH_DCGS2, PATH_DEFERRED, PATH_JVLIN, PATH_MONOTONE, PATH_BATCHED = 0, 1, 2, 4, 8, 15
+PATH_PLAIN, PATH_DCGS2, PATH_DEFERRED, PATH_JVLIN, PATH_MONOTONE, PATH_BATCHED, PATH_F32BASIS = 0, 1, 2, 4, 8, 15, 16
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libtsr.so")

@@ -186,18 +186,33 @@ hipError_t launch_lincomb(double* out, const double* base, const double* const* 
                           const double* coef, int ldc, double sgn, const BlkMap& M, int nblk, double* part, int ldp,
                           int partmode, const double* u, double at, double rt, const double* extra_sub,
                           const int* active, hipStream_t st);
+hipError_t launch_lincomb(double* out, const double* base, const float* const* cols, const int* nk_s,
+                          const double* coef, int ldc, double sgn, const BlkMap& M, int nblk, double* part, int ldp,
+                          int partmode, const double* u, double at, double rt, const double* extra_sub,
+                          const int* active, hipStream_t st);
 // out1 = base1 + sum_{k < nk1[s]} coef1 col_k, out2 = sum_{k < nk2[s]} coef2 col_k, one pass
 hipError_t launch_lincomb2(double* out1, const double* base1, double* out2, const double* const* cols,
+                           const int* nk1_s, const double* coef1, const int* nk2_s, const double* coef2, int ldc,
+                           const BlkMap& M, int nblk, const int* active, hipStream_t st);
+hipError_t launch_lincomb2(double* out1, const double* base1, double* out2, const float* const* cols,
                            const int* nk1_s, const double* coef1, const int* nk2_s, const double* coef2, int ldc,
                            const BlkMap& M, int nblk, const int* active, hipStream_t st);
 hipError_t launch_reduce(int mode, const double* part, int ldp, int nk, const int* blk_first, int n_scen,
                          const long long* scen_rows, KrylovState K, int j, const int* active, hipStream_t st);
 // One Arnoldi orthogonalisation by DCGS2 (nk = j + 1 columns; w becomes raw column j+1
-// and column j is corrected in place): k_pass_dca, column reduction, k_dc_finish,
-// k_pass_dcb, then the RED_NORM bookkeeping.
-hipError_t launch_dcgs2(int nk, const double* const* cols, double* w, const double* u, KrylovState K, const BlkMap& M,
-                       int nblk, double* part, int ldp, const int* blk_first, int n_scen, const long long* scen_rows,
-                       const int* active, double at, double rt, hipStream_t st);
+// and column j, Rj = cols[j], is corrected in place): k_pass_dca, column reduction,
+// k_dc_finish, k_pass_dcb, then the RED_NORM bookkeeping.
+hipError_t launch_dcgs2(int nk, const double* const* cols, double* Rj, double* w, const double* u, KrylovState K,
+                        const BlkMap& M, int nblk, double* part, int ldp, const int* blk_first, int n_scen,
+                        const long long* scen_rows, const int* active, double at, double rt, hipStream_t st);
+// The same with the basis read from FP32 copies: columns k < j from cols32, R_j from its
+// FP64 column Rj (left as it is), the corrected R_j written to its FP32 copy Rj32 (for
+// j = 0 too, so column 0 gets its copy).
+hipError_t launch_dcgs2_f32(int nk, const float* const* cols32, const double* Rj, float* Rj32, double* w,
+                            const double* u, KrylovState K, const BlkMap& M, int nblk, double* part, int ldp,
+                            const int* blk_first, int n_scen, const long long* scen_rows, const int* active, double at,
+                            double rt, hipStream_t st);
+hipError_t launch_to_f32(float* dst, const double* src, const BlkMap& M, int nblk, const int* mask, hipStream_t st);
 hipError_t launch_masked_copy(double* dst, const double* src, const BlkMap& M, int nblk, const int* mask,
                               hipStream_t st);
 // Readbacks into a mapped pinned host buffer (layouts in krylov.hip); both clear the

@@ -149,8 +149,11 @@ __global__ __launch_bounds__(kVecBS) void k_pass(int mode, const double* const* 
 //   part[nk + k][b]   = sum R_k R_j    (k < j)
 //   part[2 nk - 1][b] = sum R_j R_j
 //   part[2 nk][b]     = sum w w
-__global__ __launch_bounds__(kVecBS) void k_pass_dca(const double* const* cols, int nk, const double* w, BlkMap M,
-                                                    double* part, int ldp, const int* active)
+// Columns k < j are read as T (FP64, or the FP32 copies of TSR_PATH_F32BASIS); R_j, the
+// once-projected vector of the previous iteration, is always read in FP64.
+template <class T>
+__global__ __launch_bounds__(kVecBS) void k_pass_dca(const T* const* cols, const double* Rj, int nk, const double* w,
+                                                    BlkMap M, double* part, int ldp, const int* active)
 {
     __shared__ double lds[kVecBS / 64][2 * kMaxCols + 2];
     const int b = blockIdx.x;
@@ -162,7 +165,6 @@ __global__ __launch_bounds__(kVecBS) void k_pass_dca(const double* const* cols, 
     double wr[kVecR], rj[kVecR];
     long long idx[kVecR];
     bool ok[kVecR];
-    const double* Rj = cols[j];
 #pragma unroll
     for (int r = 0; r < kVecR; ++r) {
         idx[r] = r0 + tid + (long long)r * kVecBS;
@@ -178,7 +180,7 @@ __global__ __launch_bounds__(kVecBS) void k_pass_dca(const double* const* cols, 
             if (ok[r]) {
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    const double a = cols[k + c][idx[r]];
+                    const double a = (double)cols[k + c][idx[r]];
                     sx[c] += a * wr[r];
                     px[c] += a * rj[r];
                 }
@@ -194,11 +196,11 @@ __global__ __launch_bounds__(kVecBS) void k_pass_dca(const double* const* cols, 
         }
     }
     for (; k < j; ++k) {
-        const double* c0 = cols[k];
+        const T* c0 = cols[k];
         double s0 = 0.0, p0 = 0.0;
 #pragma unroll
         for (int r = 0; r < kVecR; ++r)
-            if (ok[r]) { const double a0 = c0[idx[r]]; s0 += a0 * wr[r]; p0 += a0 * rj[r]; }
+            if (ok[r]) { const double a0 = (double)c0[idx[r]]; s0 += a0 * wr[r]; p0 += a0 * rj[r]; }
         s0 = wave_sum_v(s0); p0 = wave_sum_v(p0);
         if (lane == 0) { lds[wv][k] = s0; lds[wv][nk + k] = p0; }
     }
@@ -282,10 +284,14 @@ __global__ void k_dc_finish(KrylovState K, int j, int n_scen, const int* active)
 }
 
 // w -= sum_{k<=j} coef_k R_k;  R_j -= sum_{k<j} coefr_k R_k (both from the old R_j);
-// part[0][b] = sum w'^2, part[1][b] = sum (w'/wt(u))^2.
-__global__ __launch_bounds__(kVecBS) void k_pass_dcb(const double* const* cols, int nk, double* w, const double* u,
-                                                    const double* coef, const double* coefr, int ldc, BlkMap M,
-                                                    double* part, int ldp, const int* active, double at, double rt)
+// part[0][b] = sum w'^2, part[1][b] = sum (w'/wt(u))^2.  The old R_j is read from Rj_in
+// (FP64); the corrected one goes to Rj_out (for j > 0, or for every j when all_j: the
+// FP32 copy of column 0 is written here too).
+template <class T>
+__global__ __launch_bounds__(kVecBS) void k_pass_dcb(const T* const* cols, const double* Rj_in, T* Rj_out, int all_j,
+                                                    int nk, double* w, const double* u, const double* coef,
+                                                    const double* coefr, int ldc, BlkMap M, double* part, int ldp,
+                                                    const int* active, double at, double rt)
 {
     __shared__ double red[kVecBS / 64];
     const int b = blockIdx.x;
@@ -296,7 +302,6 @@ __global__ __launch_bounds__(kVecBS) void k_pass_dcb(const double* const* cols, 
     const int j = nk - 1;
     const double* cf = coef + (size_t)s * ldc;
     const double* cr = coefr + (size_t)s * ldc;
-    double* Rj = const_cast<double*>(cols[j]);
     double wr[kVecR], rj[kVecR], rn[kVecR];
     long long idx[kVecR];
     bool ok[kVecR];
@@ -305,20 +310,21 @@ __global__ __launch_bounds__(kVecBS) void k_pass_dcb(const double* const* cols, 
     for (int r = 0; r < kVecR; ++r) {
         idx[r] = r0 + tid + (long long)r * kVecBS;
         ok[r] = idx[r] < r1;
-        rj[r] = ok[r] ? Rj[idx[r]] : 0.0;
+        rj[r] = ok[r] ? Rj_in[idx[r]] : 0.0;
         wr[r] = ok[r] ? w[idx[r]] - cj * rj[r] : 0.0;
         rn[r] = rj[r];
     }
     int k = 0;
     for (; k + 4 <= j; k += 4) {   // four columns at a time: 16 loads in flight per thread
-        const double* c0 = cols[k];
-        const double* c1 = cols[k + 1];
-        const double* c2p = cols[k + 2];
-        const double* c3 = cols[k + 3];
+        const T* c0 = cols[k];
+        const T* c1 = cols[k + 1];
+        const T* c2p = cols[k + 2];
+        const T* c3 = cols[k + 3];
 #pragma unroll
         for (int r = 0; r < kVecR; ++r)
             if (ok[r]) {
-                const double a0 = c0[idx[r]], a1 = c1[idx[r]], a2 = c2p[idx[r]], a3 = c3[idx[r]];
+                const double a0 = (double)c0[idx[r]], a1 = (double)c1[idx[r]], a2 = (double)c2p[idx[r]],
+                             a3 = (double)c3[idx[r]];
                 wr[r] -= a0 * cf[k];
                 rn[r] -= a0 * cr[k];
                 wr[r] -= a1 * cf[k + 1];
@@ -331,11 +337,11 @@ __global__ __launch_bounds__(kVecBS) void k_pass_dcb(const double* const* cols, 
     }
     for (; k < j; ++k) {
         const double c = cf[k], c2 = cr[k];
-        const double* col = cols[k];
+        const T* col = cols[k];
 #pragma unroll
         for (int r = 0; r < kVecR; ++r)
             if (ok[r]) {
-                const double a = col[idx[r]];
+                const double a = (double)col[idx[r]];
                 wr[r] -= a * c;
                 rn[r] -= a * c2;
             }
@@ -345,7 +351,7 @@ __global__ __launch_bounds__(kVecBS) void k_pass_dcb(const double* const* cols, 
     for (int r = 0; r < kVecR; ++r)
         if (ok[r]) {
             w[idx[r]] = wr[r];
-            if (j > 0) Rj[idx[r]] = rn[r];
+            if (all_j || j > 0) Rj_out[idx[r]] = (T)rn[r];
             s0 += wr[r] * wr[r];
             const double q = wr[r] / (at + rt * fabs(u[idx[r]]));
             s1 += q * q;
@@ -385,7 +391,8 @@ __global__ __launch_bounds__(kVecBS) void k_norms(const double* a, const double*
 // out = base + sgn * sum_{k < nk_s[s]} col_k * coef[s][k] - extra_sub
 // partmode 0: nothing; 1: part[0][b] = sum out^2; 2: part[0][b] = sum (c/wt(u))^2 of the
 // combination c = sum col_k coef_k alone (the error vector) with out = base + c.
-__global__ __launch_bounds__(kVecBS) void k_lincomb(double* out, const double* base, const double* const* cols,
+template <class T>
+__global__ __launch_bounds__(kVecBS) void k_lincomb(double* out, const double* base, const T* const* cols,
                                                    const int* nk_s, const double* coef, int ldc, double sgn, BlkMap M,
                                                    double* part, int ldp, int partmode, const double* u, double at,
                                                    double rt, const double* extra_sub, const int* active)
@@ -408,10 +415,10 @@ __global__ __launch_bounds__(kVecBS) void k_lincomb(double* out, const double* b
     }
     for (int k = 0; k < nk; ++k) {
         const double c = cf[k];
-        const double* col = cols[k];
+        const T* col = cols[k];
 #pragma unroll
         for (int r = 0; r < kVecR; ++r)
-            if (ok[r]) acc[r] += col[idx[r]] * c;
+            if (ok[r]) acc[r] += (double)col[idx[r]] * c;
     }
     double s0 = 0.0;
 #pragma unroll
@@ -434,8 +441,9 @@ __global__ __launch_bounds__(kVecBS) void k_lincomb(double* out, const double* b
 // pass over the columns: the two combinations of an EXPREM step that share the basis
 // (U2 and the Arnoldi-relation term of D).  Each sum runs over increasing k as in
 // k_lincomb, so both outputs equal two separate k_lincomb calls bit for bit.
+template <class T>
 __global__ __launch_bounds__(kVecBS) void k_lincomb2(double* out1, const double* base1, double* out2,
-                                                    const double* const* cols, const int* nk1_s, const double* coef1,
+                                                    const T* const* cols, const int* nk1_s, const double* coef1,
                                                     const int* nk2_s, const double* coef2, int ldc, BlkMap M,
                                                     const int* active)
 {
@@ -458,13 +466,13 @@ __global__ __launch_bounds__(kVecBS) void k_lincomb2(double* out1, const double*
         a2[r] = 0.0;
     }
     for (int k = 0; k < nk; ++k) {
-        const double* col = cols[k];
+        const T* col = cols[k];
         const bool u1 = k < nk1, u2 = k < nk2;
         const double c1 = u1 ? cf1[k] : 0.0, c2 = u2 ? cf2[k] : 0.0;
 #pragma unroll
         for (int r = 0; r < kVecR; ++r)
             if (ok[r]) {
-                const double x = col[idx[r]];
+                const double x = (double)col[idx[r]];
                 if (u1) a1[r] += x * c1;
                 if (u2) a2[r] += x * c2;
             }
@@ -476,6 +484,20 @@ __global__ __launch_bounds__(kVecBS) void k_lincomb2(double* out1, const double*
         o += a1[r];
         out1[idx[r]] = o;
         out2[idx[r]] = a2[r];
+    }
+}
+
+// FP32 copy of a column's rows (scenarios with mask set)
+__global__ __launch_bounds__(kVecBS) void k_to_f32(float* dst, const double* src, BlkMap M, const int* mask)
+{
+    const int b = blockIdx.x;
+    const int s = M.scen[b];
+    if (mask && !mask[s]) return;
+    const long long r0 = M.r0[b], r1 = M.r1[b];
+#pragma unroll
+    for (int r = 0; r < kVecR; ++r) {
+        const long long i = r0 + threadIdx.x + (long long)r * kVecBS;
+        if (i < r1) dst[i] = (float)src[i];
     }
 }
 
@@ -727,8 +749,18 @@ hipError_t launch_lincomb(double* out, const double* base, const double* const* 
                           int partmode, const double* u, double at, double rt, const double* extra_sub,
                           const int* active, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_lincomb, dim3(nblk), dim3(kVecBS), 0, st, out, base, cols, nk_s, coef, ldc, sgn, M, part, ldp,
-                       partmode, u, at, rt, extra_sub, active);
+    hipLaunchKernelGGL(k_lincomb<double>, dim3(nblk), dim3(kVecBS), 0, st, out, base, cols, nk_s, coef, ldc, sgn, M,
+                       part, ldp, partmode, u, at, rt, extra_sub, active);
+    return hipGetLastError();
+}
+
+hipError_t launch_lincomb(double* out, const double* base, const float* const* cols, const int* nk_s,
+                          const double* coef, int ldc, double sgn, const BlkMap& M, int nblk, double* part, int ldp,
+                          int partmode, const double* u, double at, double rt, const double* extra_sub,
+                          const int* active, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_lincomb<float>, dim3(nblk), dim3(kVecBS), 0, st, out, base, cols, nk_s, coef, ldc, sgn, M,
+                       part, ldp, partmode, u, at, rt, extra_sub, active);
     return hipGetLastError();
 }
 
@@ -736,8 +768,17 @@ hipError_t launch_lincomb2(double* out1, const double* base1, double* out2, cons
                            const int* nk1_s, const double* coef1, const int* nk2_s, const double* coef2, int ldc,
                            const BlkMap& M, int nblk, const int* active, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_lincomb2, dim3(nblk), dim3(kVecBS), 0, st, out1, base1, out2, cols, nk1_s, coef1, nk2_s, coef2,
-                       ldc, M, active);
+    hipLaunchKernelGGL(k_lincomb2<double>, dim3(nblk), dim3(kVecBS), 0, st, out1, base1, out2, cols, nk1_s, coef1, nk2_s,
+                       coef2, ldc, M, active);
+    return hipGetLastError();
+}
+
+hipError_t launch_lincomb2(double* out1, const double* base1, double* out2, const float* const* cols,
+                           const int* nk1_s, const double* coef1, const int* nk2_s, const double* coef2, int ldc,
+                           const BlkMap& M, int nblk, const int* active, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_lincomb2<float>, dim3(nblk), dim3(kVecBS), 0, st, out1, base1, out2, cols, nk1_s, coef1, nk2_s,
+                       coef2, ldc, M, active);
     return hipGetLastError();
 }
 
@@ -752,19 +793,44 @@ hipError_t launch_reduce(int mode, const double* part, int ldp, int nk, const in
     return hipGetLastError();
 }
 
-hipError_t launch_dcgs2(int nk, const double* const* cols, double* w, const double* u, KrylovState K, const BlkMap& M,
-                       int nblk, double* part, int ldp, const int* blk_first, int n_scen, const long long* scen_rows,
-                       const int* active, double at, double rt, hipStream_t st)
+template <class T>
+static hipError_t dcgs2(int nk, const T* const* cols, const double* Rj_in, T* Rj_out, int all_j, double* w,
+                        const double* u, KrylovState K, const BlkMap& M, int nblk, double* part, int ldp,
+                        const int* blk_first, int n_scen, const long long* scen_rows, const int* active, double at,
+                        double rt, hipStream_t st)
 {
     const int j = nk - 1;
-    hipLaunchKernelGGL(k_pass_dca, dim3(nblk), dim3(kVecBS), 0, st, cols, nk, w, M, part, ldp, active);
+    hipLaunchKernelGGL(k_pass_dca<T>, dim3(nblk), dim3(kVecBS), 0, st, cols, Rj_in, nk, w, M, part, ldp, active);
     hipLaunchKernelGGL(k_reduce, dim3(n_scen, 2 * nk + 1), dim3(256), 0, st, (int)RED_COLS, part, ldp, 2 * nk + 1,
                        blk_first, scen_rows, K, j, active);
     hipLaunchKernelGGL(k_dc_finish, dim3(n_scen), dim3(64), 0, st, K, j, n_scen, active);
-    hipLaunchKernelGGL(k_pass_dcb, dim3(nblk), dim3(kVecBS), 0, st, cols, nk, w, u, K.coef, K.coefr, K.ld, M, part, ldp,
-                       active, at, rt);
+    hipLaunchKernelGGL(k_pass_dcb<T>, dim3(nblk), dim3(kVecBS), 0, st, cols, Rj_in, Rj_out, all_j, nk, w, u, K.coef,
+                       K.coefr, K.ld, M, part, ldp, active, at, rt);
     hipLaunchKernelGGL(k_reduce, dim3(n_scen, 2), dim3(256), 0, st, (int)RED_NORM, part, ldp, 2, blk_first, scen_rows, K,
                        j, active);
+    return hipGetLastError();
+}
+
+hipError_t launch_dcgs2(int nk, const double* const* cols, double* Rj, double* w, const double* u, KrylovState K,
+                        const BlkMap& M, int nblk, double* part, int ldp, const int* blk_first, int n_scen,
+                        const long long* scen_rows, const int* active, double at, double rt, hipStream_t st)
+{
+    return dcgs2<double>(nk, cols, Rj, Rj, 0, w, u, K, M, nblk, part, ldp, blk_first, n_scen, scen_rows, active, at, rt,
+                         st);
+}
+
+hipError_t launch_dcgs2_f32(int nk, const float* const* cols32, const double* Rj, float* Rj32, double* w,
+                            const double* u, KrylovState K, const BlkMap& M, int nblk, double* part, int ldp,
+                            const int* blk_first, int n_scen, const long long* scen_rows, const int* active, double at,
+                            double rt, hipStream_t st)
+{
+    return dcgs2<float>(nk, cols32, Rj, Rj32, 1, w, u, K, M, nblk, part, ldp, blk_first, n_scen, scen_rows, active, at,
+                        rt, st);
+}
+
+hipError_t launch_to_f32(float* dst, const double* src, const BlkMap& M, int nblk, const int* mask, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_to_f32, dim3(nblk), dim3(kVecBS), 0, st, dst, src, M, mask);
     return hipGetLastError();
 }
 

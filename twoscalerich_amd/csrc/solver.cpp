@@ -90,6 +90,10 @@ struct tsr_solver {
     std::vector<double*> V, W;
     const double** Vd = nullptr;
     const double** Wd = nullptr;
+    // TSR_PATH_F32BASIS: FP32 copies of the basis columns (allocated by set_path)
+    std::vector<float*> V32, W32;
+    const float** V32d = nullptr;
+    const float** W32d = nullptr;
     // blocks
     int nblk = 0;
     BlkMap M{};
@@ -301,6 +305,9 @@ int arnoldi(tsr_solver* s, int which)
     const int S = s->S;
     std::vector<double*>& B = which == 0 ? s->V : s->W;
     const double** Bd = which == 0 ? s->Vd : s->Wd;
+    std::vector<float*>& B32 = which == 0 ? s->V32 : s->W32;
+    const float** B32d = which == 0 ? s->V32d : s->W32d;
+    const bool f32 = (s->path & TSR_PATH_F32BASIS) != 0;
     KrylovState& K = which == 0 ? s->KV : s->KW;
     const int ldh = s->mcap + 1;
     const bool dc = (s->path & TSR_PATH_DCGS2) != 0;
@@ -328,8 +335,11 @@ int arnoldi(tsr_solver* s, int which)
         const int j0 = j;
         for (; j < jt; ++j) {
             if (G_call(s, B[j], K.veps, s->g0, K.ieps, B[j + 1], s->mask)) return 2;
-            if (dc) {
-                HIPCHK(launch_dcgs2(j + 1, Bd, B[j + 1], s->u, K, s->M, s->nblk, s->part, s->ldp, s->blk_first, S,
+            if (dc && f32) {
+                HIPCHK(launch_dcgs2_f32(j + 1, B32d, B[j], B32[j], B[j + 1], s->u, K, s->M, s->nblk, s->part, s->ldp,
+                                        s->blk_first, S, s->scen_rows, s->mask, at, rt, s->st));
+            } else if (dc) {
+                HIPCHK(launch_dcgs2(j + 1, Bd, B[j], B[j + 1], s->u, K, s->M, s->nblk, s->part, s->ldp, s->blk_first, S,
                                     s->scen_rows, s->mask, at, rt, s->st));
             } else {
                 HIPCHK(launch_pass(PASS_DOTS, Bd, j + 1, B[j + 1], s->u, K.coef, K.ld, s->M, s->nblk, s->part, s->ldp,
@@ -344,6 +354,9 @@ int arnoldi(tsr_solver* s, int which)
             if (!dc)
                 HIPCHK(launch_reduce(RED_NORM, s->part, s->ldp, 2, s->blk_first, S, s->scen_rows, K, j, s->mask, s->st));
         }
+        // FP32 copy of the window's last column as it stands (the next window's first
+        // iteration corrects it and rewrites the copy; a scenario that stops here uses it)
+        if (dc && f32) HIPCHK(launch_to_f32(B32[jt], B[jt], s->M, s->nblk, s->mask, s->st));
         // H columns c0..jt-1 changed in this window (DCGS2 also corrects column j0-1)
         const int c0 = dc ? std::max(j0 - 1, 0) : j0;
         if (read_window(s, K, c0, jt)) return 2;
@@ -553,8 +566,12 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
         HIPCHK(hipMemcpyAsync(s->coef2, s->coef2_h.data(), sizeof(double) * S * s->KV.ld, hipMemcpyHostToDevice, s->st));
         HIPCHK(hipMemcpyAsync(s->nk2, s->nk2_h.data(), sizeof(int) * S, hipMemcpyHostToDevice, s->st));
         // u2 = u + V y and p2 = V_{m+1} Hbar y
-        HIPCHK(launch_lincomb2(s->u2, s->u, s->p2, s->Vd, s->nk, s->KV.coef, s->nk2, s->coef2, s->KV.ld, s->M, s->nblk,
-                               s->mask, s->st));
+        if (s->path & TSR_PATH_F32BASIS)
+            HIPCHK(launch_lincomb2(s->u2, s->u, s->p2, s->V32d, s->nk, s->KV.coef, s->nk2, s->coef2, s->KV.ld, s->M,
+                                   s->nblk, s->mask, s->st));
+        else
+            HIPCHK(launch_lincomb2(s->u2, s->u, s->p2, s->Vd, s->nk, s->KV.coef, s->nk2, s->coef2, s->KV.ld, s->M,
+                                   s->nblk, s->mask, s->st));
         if (G_at(s, s->u2, s->g2, s->mask)) return -1;   // bad flags cleared by the last readback
         // D = g2 - p2 - g0 (the one-column combination {p2} with coefficient 1)
         HIPCHK(launch_lincomb(s->D, s->g2, s->p2col, s->one_nk, s->one_coef, 1, -1.0, s->M, s->nblk, s->part, s->ldp, 1,
@@ -616,8 +633,12 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
         if (upload_mask(s)) return -1;
         HIPCHK(hipMemcpyAsync(s->KW.coef, s->coef_h.data(), sizeof(double) * S * s->KW.ld, hipMemcpyHostToDevice, s->st));
         HIPCHK(hipMemcpyAsync(s->nk, s->nk_h.data(), sizeof(int) * S, hipMemcpyHostToDevice, s->st));
-        HIPCHK(launch_lincomb(s->unew, s->u2, s->Wd, s->nk, s->KW.coef, s->KW.ld, 1.0, s->M, s->nblk, s->part, s->ldp, 2,
-                              s->u, o.AbsTol, o.RelTol, nullptr, s->mask, s->st));
+        if (s->path & TSR_PATH_F32BASIS)
+            HIPCHK(launch_lincomb(s->unew, s->u2, s->W32d, s->nk, s->KW.coef, s->KW.ld, 1.0, s->M, s->nblk, s->part,
+                                  s->ldp, 2, s->u, o.AbsTol, o.RelTol, nullptr, s->mask, s->st));
+        else
+            HIPCHK(launch_lincomb(s->unew, s->u2, s->Wd, s->nk, s->KW.coef, s->KW.ld, 1.0, s->M, s->nblk, s->part,
+                                  s->ldp, 2, s->u, o.AbsTol, o.RelTol, nullptr, s->mask, s->st));
         HIPCHK(launch_reduce(RED_SUM, s->part, s->ldp, 1, s->blk_first, S, s->scen_rows, s->KW, 0, s->mask, s->st));
         if (read_iteration(s, s->KW)) return -1;
         std::vector<int> acc(S, 0);
@@ -980,6 +1001,9 @@ void tsr_solver_free(tsr_solver* s)
     for (double* p : {s->u, s->g0, s->u2, s->g2, s->D, s->unew, s->scratch[0], s->scratch[1]}) if (p) hipFree(p);
     for (size_t k = 1; k < s->V.size(); ++k) { if (s->V[k]) hipFree(s->V[k]); if (s->W[k]) hipFree(s->W[k]); }
     hipFree(s->Vd); hipFree(s->Wd);
+    for (auto* B32 : {&s->V32, &s->W32})
+        for (float* c : *B32) if (c) hipFree(c);
+    hipFree(s->V32d); hipFree(s->W32d);
     hipFree((void*)s->M.scen); hipFree((void*)s->M.r0); hipFree((void*)s->M.r1);
     hipFree(s->blk_first); hipFree(s->scen_rows); hipFree(s->part);
     for (KrylovState* K : {&s->KV, &s->KW}) {
@@ -1197,7 +1221,36 @@ int tsr_solver_jv(tsr_solver* s, const double* v_host, double eps, int lin, doub
 
 int tsr_solver_set_path(tsr_solver* s, int path)
 {
-    if (path & ~TSR_PATH_BATCHED) return 1;
+    if (path & ~(TSR_PATH_BATCHED | TSR_PATH_F32BASIS)) return 1;
+    if ((path & TSR_PATH_F32BASIS) && !(path & TSR_PATH_DCGS2)) {
+        set_error("TSR_PATH_F32BASIS needs TSR_PATH_DCGS2");
+        return 1;
+    }
+    if ((path & TSR_PATH_F32BASIS) && !s->V32d) {
+        // columns 0..mcap of both bases
+        const size_t N = (size_t)s->N;
+        for (auto* B32 : {&s->V32, &s->W32}) {
+            B32->assign(s->mcap + 1, nullptr);
+            for (auto& c : *B32)
+                if (hipMalloc((void**)&c, sizeof(float) * (N ? N : 1)) != hipSuccess) {
+                    set_error("hipMalloc failed for the FP32 basis");
+                    return 2;
+                }
+        }
+        const float** d[2] = {nullptr, nullptr};
+        for (int w = 0; w < 2; ++w) {
+            const std::vector<float*>& B32 = w == 0 ? s->V32 : s->W32;
+            if (hipMalloc((void**)&d[w], sizeof(float*) * (s->mcap + 1)) != hipSuccess ||
+                hipMemcpy(d[w], B32.data(), sizeof(float*) * (s->mcap + 1), hipMemcpyHostToDevice) != hipSuccess) {
+                if (d[w]) hipFree(d[w]);
+                if (w == 1) hipFree(d[0]);
+                set_error("hipMalloc failed for the FP32 basis");
+                return 2;
+            }
+        }
+        s->V32d = d[0];
+        s->W32d = d[1];
+    }
     if ((path & TSR_PATH_JVLIN) && !lin_mode_ok(s->P)) {
         set_error("linearised FD products need micro templates of at most 1024 nodes");
         return 1;
