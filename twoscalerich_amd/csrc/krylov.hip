@@ -525,10 +525,12 @@ __global__ __launch_bounds__(kVecBS) void k_masked_copy(double* dst, const doubl
 //             (k = 2) res[3] = ||u||_2; eps = sqrt(DBL_EPSILON)(1 + ||u||_2) unless j < 0; ieps = 1/eps;
 //             then k_start_finish sets veps = eps cs[0]
 //   RED_SUM:  res[k] = sum_k
-__global__ __launch_bounds__(256) void k_reduce(int mode, const double* part, int ldp, int nk, const int* blk_first,
+constexpr int kRedBS = 1024;   // threads of a k_reduce workgroup
+
+__global__ __launch_bounds__(kRedBS) void k_reduce(int mode, const double* part, int ldp, int nk, const int* blk_first,
                                                 const long long* scen_rows, KrylovState K, int j, const int* active)
 {
-    __shared__ double red[256 / 64];
+    __shared__ double red[kRedBS / 64];
     const int s = blockIdx.x;
     const int k = blockIdx.y;
     if (active && !active[s]) return;
@@ -536,15 +538,15 @@ __global__ __launch_bounds__(256) void k_reduce(int mode, const double* part, in
     if (k >= nsum) return;
     const int b0 = blk_first[s], b1 = blk_first[s + 1];
     const double* col = part + (size_t)k * ldp;
-    // thread t takes partials t, t + 256, ...: eight running sums (partials 8i + r go to
-    // sum r), so eight loads are in flight per thread, combined in a fixed tree
+    // thread t takes partials t, t + kRedBS, ...: eight running sums (partials 8i + r go
+    // to sum r), so eight loads are in flight per thread, combined in a fixed tree
     double xs[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     int b = b0 + threadIdx.x;
-    for (; b + 7 * 256 < b1; b += 8 * 256) {
+    for (; b + 7 * kRedBS < b1; b += 8 * kRedBS) {
 #pragma unroll
-        for (int r = 0; r < 8; ++r) xs[r] += col[b + r * 256];
+        for (int r = 0; r < 8; ++r) xs[r] += col[b + r * kRedBS];
     }
-    for (int r = 0; b < b1; b += 256, ++r) xs[r] += col[b];
+    for (int r = 0; b < b1; b += kRedBS, ++r) xs[r] += col[b];
     double x = ((xs[0] + xs[1]) + (xs[2] + xs[3])) + ((xs[4] + xs[5]) + (xs[6] + xs[7]));
     x = wave_sum_v(x);
     const int tid = threadIdx.x;
@@ -552,7 +554,7 @@ __global__ __launch_bounds__(256) void k_reduce(int mode, const double* part, in
     __syncthreads();
     if (tid != 0) return;
     double acc = 0.0;
-    for (int w = 0; w < 4; ++w) acc += red[w];
+    for (int w = 0; w < kRedBS / 64; ++w) acc += red[w];
     double* cs = K.cs + (size_t)s * K.ld;
     double* coef = K.coef + (size_t)s * K.ld;
     double* H = K.H + (size_t)s * K.hstride;
@@ -786,7 +788,7 @@ hipError_t launch_reduce(int mode, const double* part, int ldp, int nk, const in
                          const long long* scen_rows, KrylovState K, int j, const int* active, hipStream_t st)
 {
     const int ncol = (mode == RED_H1 || mode == RED_H2) ? nk + 1 : nk;
-    hipLaunchKernelGGL(k_reduce, dim3(n_scen, ncol), dim3(256), 0, st, mode, part, ldp, nk, blk_first, scen_rows, K, j,
+    hipLaunchKernelGGL(k_reduce, dim3(n_scen, ncol), dim3(kRedBS), 0, st, mode, part, ldp, nk, blk_first, scen_rows, K, j,
                        active);
     if (mode == RED_START)
         hipLaunchKernelGGL(k_start_finish, dim3((n_scen + 255) / 256), dim3(256), 0, st, K, n_scen, active);
@@ -801,12 +803,12 @@ static hipError_t dcgs2(int nk, const T* const* cols, const double* Rj_in, T* Rj
 {
     const int j = nk - 1;
     hipLaunchKernelGGL(k_pass_dca<T>, dim3(nblk), dim3(kVecBS), 0, st, cols, Rj_in, nk, w, M, part, ldp, active);
-    hipLaunchKernelGGL(k_reduce, dim3(n_scen, 2 * nk + 1), dim3(256), 0, st, (int)RED_COLS, part, ldp, 2 * nk + 1,
+    hipLaunchKernelGGL(k_reduce, dim3(n_scen, 2 * nk + 1), dim3(kRedBS), 0, st, (int)RED_COLS, part, ldp, 2 * nk + 1,
                        blk_first, scen_rows, K, j, active);
     hipLaunchKernelGGL(k_dc_finish, dim3(n_scen), dim3(64), 0, st, K, j, n_scen, active);
     hipLaunchKernelGGL(k_pass_dcb<T>, dim3(nblk), dim3(kVecBS), 0, st, cols, Rj_in, Rj_out, all_j, nk, w, u, K.coef,
                        K.coefr, K.ld, M, part, ldp, active, at, rt);
-    hipLaunchKernelGGL(k_reduce, dim3(n_scen, 2), dim3(256), 0, st, (int)RED_NORM, part, ldp, 2, blk_first, scen_rows, K,
+    hipLaunchKernelGGL(k_reduce, dim3(n_scen, 2), dim3(kRedBS), 0, st, (int)RED_NORM, part, ldp, 2, blk_first, scen_rows, K,
                        j, active);
     return hipGetLastError();
 }
