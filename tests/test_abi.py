@@ -87,3 +87,20 @@ def test_gfunc_hip_rejects_unregistered_user_data_of_unknown_shape():
     assert T.lib().Gfunc_hip(T._dp(g), T._dp(u), ud) == 2
     assert b"M_nodes + M_elements*m_nodes" in T.lib().tsr_last_error()
     p.close()
+
+
+def test_exprem_c_rejects_bad_options_without_a_gpu():
+    # the solver exprem_c makes for a problem is checked before any device work; a
+    # failed set-up returns -1 (and does not hold the solver cache's lock)
+    p = T.Problem.config(F.CFG_PLAIN_SMALL)
+    o = p.options()
+    o.MaxKrylov = 500
+    u = p.initial()
+    t = C.c_double(0.0)
+    st = F.exprem_stats_struct()
+    gf = C.cast(T.lib().Gfunc_hip, C.c_void_p)
+    for _ in range(2):
+        rc = T.lib().exprem_c(gf, p.N, C.byref(t), p.tend(), None, None, T._dp(u), p.ud.contents, o, C.byref(st))
+        assert rc == -1
+    assert b"MaxKrylov" in T.lib().tsr_last_error()
+    p.close()

@@ -721,6 +721,9 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
     return 0;
 }
 
+// frees a solver's device and host resources (it must not be in the solver cache)
+static void release_solver(tsr_solver* s);
+
 namespace {
 
 // Controller inputs (16 per scenario, order of TSR_FEAT_COUNT in tsr.h) and, with the
@@ -826,7 +829,7 @@ void prefer_spin_sync()
 
 // a solver whose set-up fails part way is released like a finished one
 struct SolverRelease {
-    void operator()(tsr_solver* s) const { tsr_solver_free(s); }
+    void operator()(tsr_solver* s) const { release_solver(s); }
 };
 
 tsr_solver* create(const tsr_batch_view* view, const exprem_options_struct* opt)
@@ -997,6 +1000,14 @@ void tsr_solver_free(tsr_solver* s)
         for (auto it = g_solver_cache.begin(); it != g_solver_cache.end();)
             it = (it->second == s) ? g_solver_cache.erase(it) : std::next(it);
     }
+    release_solver(s);
+}
+
+}  // extern "C"
+
+static void release_solver(tsr_solver* s)
+{
+    if (!s) return;
     if (s->P) hipDeviceSynchronize();   // no kernel ran before the problem was uploaded
     for (double* p : {s->u, s->g0, s->u2, s->g2, s->D, s->unew, s->scratch[0], s->scratch[1]}) if (p) hipFree(p);
     for (size_t k = 1; k < s->V.size(); ++k) { if (s->V[k]) hipFree(s->V[k]); if (s->W[k]) hipFree(s->W[k]); }
@@ -1023,6 +1034,8 @@ void tsr_solver_free(tsr_solver* s)
     if (s->st) hipStreamDestroy(s->st);
     delete s;
 }
+
+extern "C" {
 
 void tsr_solver_reset(tsr_solver* s)
 {
@@ -1299,6 +1312,15 @@ int tsr_d2h(double* dst, const double* src, long long n)
 }
 
 // ---------------------------------------------------------------- reference signatures
+// field by field: the struct has padding, whose bytes a caller's copy need not repeat
+static bool same_options(const exprem_options_struct& a, const exprem_options_struct& b)
+{
+    return a.AbsTol == b.AbsTol && a.RelTol == b.RelTol && a.MinKrylov == b.MinKrylov && a.MaxKrylov == b.MaxKrylov &&
+           a.MinStep == b.MinStep && a.MaxStep == b.MaxStep && a.MaxNumSteps == b.MaxNumSteps &&
+           a.SafetyFac == b.SafetyFac && a.MinStepDecFac == b.MinStepDecFac && a.MaxStepIncFac == b.MaxStepIncFac &&
+           a.InitStep == b.InitStep;
+}
+
 static tsr_solver* solver_for(const user_data_struct& ud, const exprem_options_struct* opt)
 {
     std::string err;
@@ -1306,7 +1328,7 @@ static tsr_solver* solver_for(const user_data_struct& ud, const exprem_options_s
     if (!v) { set_error(err); return nullptr; }
     std::lock_guard<std::mutex> lk(g_cache_mu);
     auto it = g_solver_cache.find(v);
-    if (it != g_solver_cache.end() && (!opt || std::memcmp(&it->second->opt, opt, sizeof(*opt)) == 0)) return it->second;
+    if (it != g_solver_cache.end() && (!opt || same_options(it->second->opt, *opt))) return it->second;
     exprem_options_struct o;
     if (opt) o = *opt;
     else if (it != g_solver_cache.end()) o = it->second->opt;
