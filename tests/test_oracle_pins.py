@@ -513,3 +513,28 @@ def test_oracle_exprem_lands_on_tspan_and_fills_solutions():
     assert hit == [2e-4, 5e-4]
     assert np.array_equal(sol[2 * p.N:], u) and np.any(sol[p.N:2 * p.N] != 0)
     o.close()
+
+
+def test_oracle_exprem_independent_of_thread_count():
+    """The oracle's inner products sum in an order fixed by N alone, so its EXPREM steps
+    repeat bit for bit whatever the OpenMP thread count (a reduction clause would not)."""
+    gomp = C.CDLL("libgomp.so.1")
+    p = T.Problem.config(F.CFG_PLAIN_SMALL, 9, 5)
+    o = O.Oracle(p.view, p.ud)
+    opt = p.options()
+    out = []
+    try:
+        for nt in (1, 5):
+            gomp.omp_set_num_threads(nt)
+            u = p.initial()
+            st = F.exprem_stats_struct()
+            t = 0.0
+            for _ in range(4):
+                rc, t, _ = o.exprem_step(p.ud.contents, u, t, p.tend(), opt, st)
+                assert rc == 0
+            out.append((t, u.copy()))
+    finally:
+        gomp.omp_set_num_threads(os.cpu_count() or 1)
+    assert out[0][0] == out[1][0] and np.array_equal(out[0][1], out[1][1])
+    o.close()
+    p.close()
