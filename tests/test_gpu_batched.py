@@ -77,6 +77,26 @@ def test_scenario_batch_matches_single_scenarios():
     pb.close()
 
 
+def test_scenario_batch_on_fp32_basis():
+    """The FP32 basis copies with scenarios that stop their Arnoldi windows at different
+    dimensions (each keeps the copy of the column it stopped at): every scenario of the
+    batch stays within FP32 rounding of the FP64-basis batched path."""
+    S = 4
+    pb = T.Problem.config(F.CFG_SCENARIOS, macro_n=21, micro_r=11, n_scen=S)
+    v = pb.view.contents
+    offs = [v.scen[q].u_offset for q in range(S)] + [pb.N]
+    tend = 1e-3
+    u64, s64 = _run(pb, F.PATH_BATCHED, tend)
+    u32, s32 = _run(pb, F.PATH_BATCHED | F.PATH_F32BASIS, tend)
+    print(f"[batched] {S} scenarios: FP64 basis {s64.steps} solver steps / {s64.fevals} RHS, "
+          f"FP32 basis {s32.steps} / {s32.fevals}", flush=True)
+    for q in range(S):
+        d = _rel2(u32[offs[q]:offs[q + 1]], u64[offs[q]:offs[q + 1]])
+        print(f"[batched]   scenario {q}: FP32 vs FP64 basis {d:.2e}", flush=True)
+        assert d <= 1e-6
+    pb.close()
+
+
 def test_learned_controller_gain():
     p = T.Problem.config(F.CFG_PLAIN)
     tend = 0.1
