@@ -1,7 +1,7 @@
 """Benchmark of the TwoScaleRich time-stepping hot path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--plain-tend T] [--full-plain]
-                    [--cpu-sample-s S] [--no-cpu]
+                    [--cpu-sample-s S] [--no-cpu] [--basis f32|f64]
 
 Metric (the reference's headline for this path): wall-time to t_end of the exponential
 integrator, learned + batched GPU EXPREM against plain GPU EXPREM, on the plain GPU-scale
@@ -10,7 +10,8 @@ unknowns, t in [0, 1] day, rtol = atol = 1e-6), with the RHS throughput beside i
 
 A "step" is one pass of the hot path over the workload: the integration from t = 0 to
 t_end with the learned controller on the batched solver path (every accepted EXPREM
-step, all on the device; inputs resident in HBM before the timer starts).  `value` is
+step, all on the device; inputs resident in HBM before the timer starts; DCGS2 reads
+FP32 copies of the Krylov basis unless --basis f64, the arithmetic is FP64 either way).  `value` is
 its wall-time in seconds (lower is better).  W warm-up steps integrate a short horizon
 first (kernel loading, launch-shape tuning); K timed steps each integrate the full
 horizon from the same initial state.
