@@ -104,3 +104,26 @@ def test_exprem_c_rejects_bad_options_without_a_gpu():
         assert rc == -1
     assert b"MaxKrylov" in T.lib().tsr_last_error()
     p.close()
+
+
+def test_unregistered_user_data_hosting_rules_without_a_gpu():
+    # N = M_nodes * (1 + m_nodes) is taken as cells hosted by macro nodes (this workload's
+    # layout) and N = M_nodes + M_elements * m_nodes as cells hosted by macro elements: both
+    # pass the shape check and reach the device upload, which is what fails without a GPU
+    import numpy as np
+    import torch
+    if torch.cuda.device_count() > 0:
+        import pytest
+        pytest.skip("checks the failure past the shape check, which needs a machine without a GPU")
+    p = T.Problem.config(F.CFG_PLAIN_SMALL)
+    ud = F.user_data_struct.from_buffer_copy(p.ud.contents)
+    M, m = ud.macro_mesh, ud.micro_mesh
+    el = np.ctypeslib.as_array(M.elements, shape=(M.no_verts * M.no_elements,)).copy()
+    ud.macro_mesh.elements = el.ctypes.data_as(C.POINTER(C.c_int))
+    for N in (M.no_nodes * (1 + m.no_nodes), M.no_nodes + M.no_elements * m.no_nodes):
+        ud.N = N
+        u = np.zeros(N)
+        g = np.zeros(N)
+        assert T.lib().Gfunc_hip(T._dp(g), T._dp(u), ud) == 2
+        assert b"neither" not in T.lib().tsr_last_error()
+    p.close()
