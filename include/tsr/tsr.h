@@ -134,7 +134,14 @@ typedef struct tsr_step_record {
     int    fevals;       /* RHS evaluations in this step */
     int    controller;   /* 0 classical, 1 learned proposal, 2 learned output unusable (classical), 3 exploration */
     int    m_cap;        /* Krylov cap used */
+    double tau_next;     /* step the classical law proposes next (exprem_stats_struct.CurrentStep) */
 } tsr_step_record;
+
+/* Writes (append = 0) or appends one line per record in the reference's statistics.txt
+ * format (src/main.cpp:324-329): the step number (first_step, first_step + 1, ...) right-
+ * aligned in 5 columns, then t after the step, the step proposed next and the Krylov
+ * dimension, "%5d %.4e %.4e %d".  Returns 0, or 1 when the file cannot be opened. */
+int tsr_statistics_write(const char* path, const tsr_step_record* recs, int n, int first_step, int append);
 
 typedef struct tsr_run_stats {
     double t;
@@ -178,6 +185,25 @@ int tsr_solver_time_gfunc_mode(tsr_solver* s, double* g_dev, const double* u_dev
 /* Average device time (ms) of a plain copy of N doubles (reads 8N, writes 8N bytes):
  * the calibration kernel for HBM traffic counters. */
 int tsr_solver_time_copy(tsr_solver* s, int reps, double* ms);
+
+/* Sampled per-kernel device timing while the solver steps: HIP events on the solver's
+ * stream around every `every`-th launch of each kernel class below (0 turns it off);
+ * counters are reset.  _read gives, per class, the launches seen, the launches timed,
+ * their summed device time (ms) and summed algorithmic HBM bytes (every scenario
+ * active).  Returns 0 (1 on every < 0). */
+enum { TSR_KT_MICRO_FULL = 0,   /* k_micro, G or FD product of the full RHS */
+       TSR_KT_MICRO_LIN = 1,    /* k_micro, linearised FD product */
+       TSR_KT_MACRO_NODE = 2,   /* k_macro_node */
+       TSR_KT_DCA = 3,          /* k_pass_dca, DCGS2 projections */
+       TSR_KT_DCB = 4,          /* k_pass_dcb, DCGS2 update */
+       TSR_KT_PASS = 5,         /* k_pass, CGS2 passes of the plain path */
+       TSR_KT_COUNT = 6 };
+int tsr_solver_ktimer(tsr_solver* s, int every);
+int tsr_solver_ktimer_read(tsr_solver* s, long long* seen, long long* timed, double* ms, double* bytes);
+
+/* Step size the next step starts from (the CurrentStep of exprem_stats_struct, used when
+ * a state from elsewhere is resumed).  Returns 1 unless tau is positive and finite. */
+int tsr_solver_set_step(tsr_solver* s, double tau);
 
 /* One accepted step on the device state (the exprem contract).  rec may be NULL. */
 int tsr_solver_step(tsr_solver* s, double tend, tsr_step_record* rec);

@@ -153,6 +153,24 @@ class Solver:
     def set_controller(self, mode):
         return lib().tsr_solver_set_controller(self.h, mode)
 
+    def set_step(self, tau):
+        if lib().tsr_solver_set_step(self.h, tau):
+            raise ValueError(f"step size {tau!r} must be positive and finite")
+
+    def ktimer(self, every):
+        """Sample every `every`-th launch of each timed kernel class with HIP events (0: off)."""
+        if lib().tsr_solver_ktimer(self.h, every):
+            raise ValueError("every must be >= 0")
+
+    def ktimer_read(self):
+        """Per kernel class (_ffi.KT_NAMES): launches seen and timed, device ms and
+        algorithmic bytes of the timed launches."""
+        n = F.KT_COUNT
+        seen, timed = (C.c_longlong * n)(), (C.c_longlong * n)()
+        ms, by = (C.c_double * n)(), (C.c_double * n)()
+        lib().tsr_solver_ktimer_read(self.h, seen, timed, ms, by)
+        return {F.KT_NAMES[c]: {"seen": seen[c], "timed": timed[c], "ms": ms[c], "bytes": by[c]} for c in range(n)}
+
     def close(self):
         if self.h:
             lib().tsr_solver_free(self.h)

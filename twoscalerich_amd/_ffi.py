@@ -76,7 +76,8 @@ class tsr_step_record(C.Structure):
                                           "head_mean", "head_min", "head_max", "sat_mean", "sat_min", "sat_max",
                                           "tau_first", "err_first")] + \
                [("feat", C.c_float * 16), ("wall_ms", C.c_double)] + \
-               [(n, C.c_int) for n in ("m", "mD", "rejects", "fevals", "controller", "m_cap")]
+               [(n, C.c_int) for n in ("m", "mD", "rejects", "fevals", "controller", "m_cap")] + \
+               [("tau_next", C.c_double)]
 
 
 class tsr_run_stats(C.Structure):
@@ -90,6 +91,9 @@ GFUNC_T = C.CFUNCTYPE(C.c_int, _dp, _dp, user_data_struct)
 HOST_ELEMENT, HOST_NODE = 0, 1
 CFG_PLAIN_SMALL, CFG_PLAIN, CFG_RAGGED, CFG_SCENARIOS, CFG_LARGE = 0, 1, 2, 3, 4
 PATH_PLAIN, PATH_DCGS2, PATH_DEFERRED, PATH_JVLIN, PATH_MONOTONE, PATH_BATCHED, PATH_F32BASIS = 0, 1, 2, 4, 8, 15, 16
+KT_NAMES = ("k_micro (full RHS)", "k_micro (linearised FD product)", "k_macro_node", "k_pass_dca", "k_pass_dcb",
+            "k_pass")
+KT_COUNT = len(KT_NAMES)
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libtsr.so")
@@ -120,6 +124,7 @@ def declare(lib):
         "tsr_problem_options": (None, [P, C.POINTER(exprem_options_struct)]),
         "tsr_problem_distinct_bytes": (C.c_longlong, [P]),
         "tsr_last_error": (C.c_char_p, []),
+        "tsr_statistics_write": (C.c_int, [C.c_char_p, C.POINTER(tsr_step_record), C.c_int, C.c_int, C.c_int]),
         "tsr_template_structured": (C.c_int, [C.POINTER(micro_mesh_struct)]),
         "tsr_solver_create": (P, [C.POINTER(tsr_batch_view), C.POINTER(exprem_options_struct)]),
         "tsr_solver_free": (None, [P]),
@@ -134,6 +139,9 @@ def declare(lib):
         "tsr_solver_time_gfunc_mode": (C.c_int, [P, C.c_void_p, C.c_void_p, C.c_int, C.c_int, _dp]),
         "tsr_solver_time_copy": (C.c_int, [P, C.c_int, _dp]),
         "tsr_solver_micro_shape": (None, [P, _ip, _ip]),
+        "tsr_solver_ktimer": (C.c_int, [P, C.c_int]),
+        "tsr_solver_ktimer_read": (C.c_int, [P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _dp, _dp]),
+        "tsr_solver_set_step": (C.c_int, [P, C.c_double]),
         "tsr_solver_step": (C.c_int, [P, C.c_double, C.POINTER(tsr_step_record)]),
         "tsr_solver_run": (C.c_int, [P, C.c_double, C.c_int, C.POINTER(tsr_run_stats),
                                      C.POINTER(tsr_step_record), C.c_int]),

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libtsr.so")
 HIP_SOURCES = ["gfunc.hip", "krylov.hip", "mlp.hip"]
-CXX_SOURCES = ["mesh.cpp", "problem.cpp", "dense.cpp", "solver.cpp"]
+CXX_SOURCES = ["mesh.cpp", "problem.cpp", "dense.cpp", "solver.cpp", "refabi.cpp"]
 ARCH = os.environ.get("TSR_OFFLOAD_ARCH", "gfx950")
 
 

@@ -152,6 +152,23 @@ bool        lin_mode_ok(const DevProblem* p);
 // tuned micro-cell launch shape (bs = 0 before the first G)
 void        micro_shape(const DevProblem* p, int* bs, int* wpe);
 
+// Sampled device timing of kernel launches (the solver's measurement mode, HIP events
+// on the launch's stream).  Launchers bracket the kernels of these classes with
+// kt_begin / kt_end; kt_begin returns false, and records nothing, unless timing is on
+// for the calling thread and this launch of the class is sampled.  `bytes` is the
+// launch's algorithmic HBM traffic with every scenario active.
+enum KtClass {
+    KT_MICRO_FULL = 0,   // k_micro modes 0/1 (G, or an FD product of the full RHS)
+    KT_MICRO_LIN = 1,    // k_micro mode 2 (linearised FD product)
+    KT_MACRO_NODE = 2,   // k_macro_node
+    KT_DCA = 3,          // k_pass_dca (DCGS2 projections)
+    KT_DCB = 4,          // k_pass_dcb (DCGS2 update)
+    KT_PASS = 5,         // k_pass (CGS2 passes of the plain path)
+    KT_COUNT = 6
+};
+bool kt_begin(int cls, double bytes, hipStream_t st);
+void kt_end(hipStream_t st);
+
 // Row blocks of the vector kernels: block b covers rows [r0, r1) of scenario scen.
 struct BlkMap {
     const int* scen;

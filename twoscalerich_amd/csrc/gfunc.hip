@@ -840,6 +840,7 @@ struct DevProblem {
     int micro_wpe[3] = {1, 1, 1};
     double tune_ms[64];               // measured candidates (for reports)
     int tune_n = 0;
+    long long n_macro = 0;            // macro nodes of all scenarios (the rest of N are micro nodes)
 };
 
 namespace {
@@ -1151,6 +1152,7 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
     B.cm_sf = at<double>(base, o_cmsf); B.cm_vi = at<int>(base, o_cmvi);
     B.cm_host = at<int>(base, o_cmh); B.cm_ninc = at<unsigned char>(base, o_cmn);
     B.Ss = v.Ss; B.N = v.N;
+    for (int q = 0; q < v.n_scen; ++q) P->n_macro += v.scen[q].ud.macro_mesh.no_nodes;
     {
         const long long n = (long long)v.n_cells * nn_max, nw = (long long)v.n_cells * soil_words;
         if (n > 0) hipLaunchKernelGGL(k_dir_elem, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, B);
@@ -1292,10 +1294,18 @@ hipError_t launch_gfunc_part(const DevProblem* p, const GArgs& a, int part, hipS
         const int mode = micro_mode(a);
         if (mode == 2 && !lin_mode_ok(p)) return hipErrorInvalidValue;
         if (p->micro_bs[mode] == 0) return tune_micro(const_cast<DevProblem*>(p), a, st);
+        // algorithmic bytes: u (and v, G(u) for an FD product) read, the result written,
+        // per micro node
+        const double bpu = 8.0 * (2 + (a.v ? 1 : 0) + (a.g0 ? 1 : 0));
+        const bool kt = kt_begin(mode == 2 ? KT_MICRO_LIN : KT_MICRO_FULL, bpu * (double)(B.N - p->n_macro), st);
         launch_micro(p, a, p->micro_bs[mode], p->micro_wpe[mode], st);
+        if (kt) kt_end(st);
     } else {
         dim3 gn((B.macro_nn_max + 63) / 64, B.n_scen);
+        const double bpu = 8.0 * (2 + (a.v ? 1 : 0) + (a.g0 ? 1 : 0));
+        const bool kt = kt_begin(KT_MACRO_NODE, bpu * (double)p->n_macro, st);
         hipLaunchKernelGGL(k_macro_node, gn, dim3(64), 0, st, B, a);
+        if (kt) kt_end(st);
     }
     return hipGetLastError();
 }

@@ -734,8 +734,14 @@ hipError_t launch_pass(int mode, const double* const* cols, int nk, double* w, c
                        int ldc, const BlkMap& M, int nblk, double* part, int ldp, const int* active, double at,
                        double rt, hipStream_t st)
 {
+    // algorithmic bytes: nk columns and w read; w written back by the two axpy passes;
+    // u read by the last
+    const double rows = (double)nblk * kVecBS * kVecR;
+    const double bytes = 8.0 * rows * (nk + 1 + (mode != PASS_DOTS ? 1 : 0) + (mode == PASS_AXPY_NORM ? 1 : 0));
+    const bool kt = kt_begin(KT_PASS, bytes, st);
     hipLaunchKernelGGL(k_pass, dim3(nblk), dim3(kVecBS), 0, st, mode, cols, nk, w, u, coef, ldc, M, part, ldp, active,
                        at, rt);
+    if (kt) kt_end(st);
     return hipGetLastError();
 }
 
@@ -802,12 +808,21 @@ static hipError_t dcgs2(int nk, const T* const* cols, const double* Rj_in, T* Rj
                         double rt, hipStream_t st)
 {
     const int j = nk - 1;
+    // algorithmic bytes per row: k_pass_dca reads columns k < j (as T), R_j and w;
+    // k_pass_dcb reads them and u, writes w and (j > 0 or all_j) the corrected R_j (as T)
+    const double rows = (double)nblk * kVecBS * kVecR;
+    const double bca = rows * (sizeof(T) * (double)j + 16.0);
+    const double bcb = rows * (sizeof(T) * (double)j + 32.0 + ((all_j || j > 0) ? sizeof(T) : 0));
+    bool kt = kt_begin(KT_DCA, bca, st);
     hipLaunchKernelGGL(k_pass_dca<T>, dim3(nblk), dim3(kVecBS), 0, st, cols, Rj_in, nk, w, M, part, ldp, active);
+    if (kt) kt_end(st);
     hipLaunchKernelGGL(k_reduce, dim3(n_scen, 2 * nk + 1), dim3(kRedBS), 0, st, (int)RED_COLS, part, ldp, 2 * nk + 1,
                        blk_first, scen_rows, K, j, active);
     hipLaunchKernelGGL(k_dc_finish, dim3(n_scen), dim3(64), 0, st, K, j, n_scen, active);
+    kt = kt_begin(KT_DCB, bcb, st);
     hipLaunchKernelGGL(k_pass_dcb<T>, dim3(nblk), dim3(kVecBS), 0, st, cols, Rj_in, Rj_out, all_j, nk, w, u, K.coef,
                        K.coefr, K.ld, M, part, ldp, active, at, rt);
+    if (kt) kt_end(st);
     hipLaunchKernelGGL(k_reduce, dim3(n_scen, 2), dim3(kRedBS), 0, st, (int)RED_NORM, part, ldp, 2, blk_first, scen_rows, K,
                        j, active);
     return hipGetLastError();

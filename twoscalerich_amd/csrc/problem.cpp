@@ -309,6 +309,100 @@ const tsr_batch_view* registry_find(const user_data_struct& ud, std::string& err
 
 void set_error(const std::string& s) { g_err = s; }
 
+// The output side of save_solution (src/save_solution.cpp:40-101): a stored solution's
+// micro Dirichlet entries are not evolved (G keeps them at 0 rate), so before writing,
+// each is set to the head G uses there.  Cells hosted by macro elements take the
+// reference's formula: the macro interpolant (P1 alpha0 X + alpha1 Y + alpha2, Q1 with the
+// extra XY term) at the element centroid (model 1) or at centroid + node offset (models
+// 2, 3).  Cells hosted by macro nodes take the host's head (model 1) or the P1 interpolant
+// at the node's physical position clamped into the macro bounding box, on the incident
+// element whose smallest barycentric coordinate is largest (a later element only when
+// larger by more than 1e-12), barycentrics clipped at 0 and renormalised.  Macro nodes of
+// Dirichlet (type 5) edges hold the scenario's Dirichlet head.  Returns 0, or 1 when
+// the problem is unknown.
+int calibrate_dirichlet(const user_data_struct& ud, double* u)
+{
+    std::string err;
+    const tsr_batch_view* v = registry_find(ud, err);
+    if (!v) return 1;
+    for (int s = 0; s < v->n_scen; ++s) {
+        const tsr_scenario& sc = v->scen[s];
+        const macro_mesh_struct& M = sc.ud.macro_mesh;
+        const int nv = M.no_verts, ne = M.no_elements, nn = M.no_nodes, model = sc.ud.model_type;
+        std::vector<char> dir(nn, 0);
+        for (int e = 0; e < ne; ++e)
+            for (int p = 0; p < nv; ++p)
+                if (M.boundary_condition_edges[p * ne + e] == TSR_BC_DIRICHLET) {
+                    dir[M.elements[p * ne + e]] = 1;
+                    dir[M.elements[((p + 1) % nv) * ne + e]] = 1;
+                }
+        const double* um = u + sc.u_offset;
+        auto head = [&](int j) { return dir[j] ? sc.dirichlet_head : um[j]; };
+        // node -> incident elements, ascending (node-hosted cells)
+        std::vector<int> first(nn + 1, 0), incid;
+        double box[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
+        if (sc.host_kind == TSR_HOST_NODE) {
+            for (int e = 0; e < ne; ++e)
+                for (int p = 0; p < nv; ++p) first[M.elements[p * ne + e] + 1]++;
+            for (int j = 0; j < nn; ++j) first[j + 1] += first[j];
+            incid.resize(first[nn]);
+            std::vector<int> at(first.begin(), first.end() - 1);
+            for (int e = 0; e < ne; ++e)
+                for (int p = 0; p < nv; ++p) incid[at[M.elements[p * ne + e]]++] = e;
+            for (int j = 0; j < nn; ++j) {
+                box[0] = std::fmin(box[0], M.nodes[j]); box[1] = std::fmax(box[1], M.nodes[j]);
+                box[2] = std::fmin(box[2], M.nodes[nn + j]); box[3] = std::fmax(box[3], M.nodes[nn + j]);
+            }
+        }
+        for (int c = 0; c < sc.n_cells; ++c) {
+            const int cell = sc.first_cell + c;
+            const micro_mesh_struct& mm = v->tmpl[v->cell_tmpl[cell]];
+            const int mn = mm.no_nodes;
+            double* uc = u + v->cell_offset[cell];
+            if (sc.host_kind == TSR_HOST_ELEMENT) {
+                double a[4] = {0, 0, 0, 0};
+                for (int k = 0; k < nv; ++k)
+                    for (int i = 0; i < nv; ++i) a[k] += head(M.elements[i * ne + c]) * M.shape_funcs[(k * nv + i) * ne + c];
+                const double cx = M.centroid_elements[c], cy = M.centroid_elements[ne + c];
+                for (int i = 0; i < mn; ++i) {
+                    if (mm.boundary_nodes[i] != 1) continue;
+                    const double X = model == 1 ? cx : cx + mm.vectors[i];
+                    const double Y = model == 1 ? cy : cy + mm.vectors[mn + i];
+                    uc[i] = nv == 3 ? a[0] * X + a[1] * Y + a[2] : a[0] * X + a[1] * Y + a[2] * X * Y + a[3];
+                }
+                continue;
+            }
+            const int j = c;
+            for (int i = 0; i < mn; ++i) {
+                if (mm.boundary_nodes[i] != 1) continue;
+                if (model == 1 || nv != 3) { uc[i] = head(j); continue; }
+                const double px = std::fmin(std::fmax(M.nodes[j] + mm.vectors[i], box[0]), box[1]);
+                const double py = std::fmin(std::fmax(M.nodes[nn + j] + mm.vectors[mn + i], box[2]), box[3]);
+                int best = -1;
+                double bmin = -INFINITY, lam[3] = {0, 0, 0};
+                for (int q = first[j]; q < first[j + 1]; ++q) {
+                    const int e = incid[q];
+                    double l[3], lo = INFINITY;
+                    for (int k = 0; k < 3; ++k) {
+                        l[k] = M.shape_funcs[k * ne + e] * px + M.shape_funcs[(3 + k) * ne + e] * py +
+                               M.shape_funcs[(6 + k) * ne + e];
+                        lo = std::fmin(lo, l[k]);
+                    }
+                    if (best < 0 || lo > bmin + 1e-12) { best = e; bmin = lo; lam[0] = l[0]; lam[1] = l[1]; lam[2] = l[2]; }
+                }
+                double wsum = 0.0, hs = 0.0;
+                for (int k = 0; k < 3; ++k) {
+                    const double w = std::fmax(lam[k], 0.0);
+                    wsum += w;
+                    hs += w * head(M.elements[k * ne + best]);
+                }
+                uc[i] = hs / wsum;
+            }
+        }
+    }
+    return 0;
+}
+
 }  // namespace tsr
 
 extern "C" {
@@ -488,5 +582,15 @@ void check_step(double dt, double MinStep)
 }
 
 double check_flag_Gfunc(int flag, double dt) { return flag != 0 ? 0.5 * dt : dt; }
+
+int tsr_statistics_write(const char* path, const tsr_step_record* recs, int n, int first_step, int append)
+{
+    FILE* f = std::fopen(path, append ? "a" : "w");
+    if (!f) { g_err = std::string("cannot open ") + path; return 1; }
+    for (int k = 0; k < n; ++k)
+        std::fprintf(f, "%5d %.4e %.4e %d\n", first_step + k, recs[k].t, recs[k].tau_next, recs[k].m);
+    std::fclose(f);
+    return 0;
+}
 
 }  // extern "C"

@@ -54,6 +54,68 @@ bool  mlp_ready(const Mlp*);
 
 using namespace tsr;
 
+// Sampled per-kernel device timing (tsr_solver_ktimer): an event pair around every
+// `every`-th launch of each timed class, harvested whenever the solver has synchronised
+// its stream (every recorded pair is then complete).
+struct KTimer {
+    int every = 0;
+    long long seen[KT_COUNT] = {}, timed[KT_COUNT] = {};
+    double ms[KT_COUNT] = {}, bytes[KT_COUNT] = {};
+    struct Pend { hipEvent_t a, b; int cls; double bytes; };
+    std::vector<Pend> pend;
+    std::vector<hipEvent_t> pool, all;
+    int open = -1;
+    double open_bytes = 0.0;
+    hipEvent_t open_a = nullptr;
+};
+
+namespace tsr {
+thread_local KTimer* g_kt = nullptr;   // the timer of the solver stepping on this thread
+
+static hipEvent_t kt_event(KTimer& k)
+{
+    if (!k.pool.empty()) { hipEvent_t e = k.pool.back(); k.pool.pop_back(); return e; }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    k.all.push_back(e);
+    return e;
+}
+
+bool kt_begin(int cls, double bytes, hipStream_t st)
+{
+    KTimer* k = g_kt;
+    if (!k || k->every <= 0 || cls < 0 || cls >= KT_COUNT || k->open >= 0) return false;
+    if (k->seen[cls]++ % k->every) return false;
+    hipEvent_t a = kt_event(*k);
+    if (!a || hipEventRecord(a, st) != hipSuccess) return false;
+    k->open = cls; k->open_a = a; k->open_bytes = bytes;
+    return true;
+}
+
+void kt_end(hipStream_t st)
+{
+    KTimer* k = g_kt;
+    if (!k || k->open < 0) return;
+    hipEvent_t b = kt_event(*k);
+    if (b && hipEventRecord(b, st) == hipSuccess) k->pend.push_back({k->open_a, b, k->open, k->open_bytes});
+    else k->pool.push_back(k->open_a);
+    k->open = -1;
+}
+
+// after a stream synchronisation: every pending pair is complete
+static void kt_harvest(KTimer& k)
+{
+    for (const auto& p : k.pend) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            k.ms[p.cls] += ms; k.bytes[p.cls] += p.bytes; k.timed[p.cls]++;
+        }
+        k.pool.push_back(p.a); k.pool.push_back(p.b);
+    }
+    k.pend.clear();
+}
+}  // namespace tsr
+
 struct ScState {
     double t = 0.0, proposal = 1e-3, tau = 0.0, tstop = 0.0, tau_prop = 0.0;
     bool landing = false;
@@ -140,6 +202,7 @@ struct tsr_solver {
     double* linrec = nullptr;   // TSR_PATH_JVLIN: expansion point of the FD products (kLinBytesPerNode x N bytes)
     double t_dense = 0.0;     // host seconds in the small dense phi routines
     double t_sync = 0.0;      // host seconds waiting for the stream
+    KTimer kt;                // sampled per-kernel timing (off unless tsr_solver_ktimer)
 };
 
 namespace {
@@ -153,8 +216,17 @@ tsr_solver* g_timed = nullptr;   // solver whose dense/sync time is being accumu
 hipError_t timed_sync(tsr_solver* s)
 {
     Tic tic(s->t_sync);
-    return hipStreamSynchronize(s->st);
+    const hipError_t e = hipStreamSynchronize(s->st);
+    if (e == hipSuccess && !s->kt.pend.empty()) kt_harvest(s->kt);
+    return e;
 }
+
+// the solver's kernel timer is active on this thread while it steps
+struct KtScope {
+    KTimer* keep;
+    explicit KtScope(tsr_solver* s) : keep(g_kt) { g_kt = s->kt.every > 0 ? &s->kt : nullptr; }
+    ~KtScope() { g_kt = keep; }
+};
 double dense_dummy = 0.0;
 double& dense_acc() { return g_timed ? g_timed->t_dense : dense_dummy; }
 }
@@ -714,6 +786,7 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
             r.head_mean = c.mstat[0]; r.head_min = c.mstat[1]; r.head_max = c.mstat[2];
             r.sat_mean = c.mstat[3]; r.sat_min = c.mstat[4]; r.sat_max = c.mstat[5];
             r.tau_first = c.tau_first; r.err_first = c.err_first;
+            r.tau_next = c.proposal;
             std::memcpy(r.feat, c.feat, sizeof(r.feat));
         }
     }
@@ -1030,6 +1103,7 @@ static void release_solver(tsr_solver* s)
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
     if (s->mlp) mlp_free(s->mlp);
+    for (hipEvent_t e : s->kt.all) hipEventDestroy(e);
     free_problem(s->P);
     if (s->st) hipStreamDestroy(s->st);
     delete s;
@@ -1166,14 +1240,45 @@ int tsr_solver_time_copy(tsr_solver* s, int reps, double* ms)
     return 0;
 }
 
+int tsr_solver_ktimer(tsr_solver* s, int every)
+{
+    if (every < 0) return 1;
+    if (hipStreamSynchronize(s->st) == hipSuccess && !s->kt.pend.empty()) kt_harvest(s->kt);
+    KTimer& k = s->kt;
+    k.every = every;
+    for (int c = 0; c < KT_COUNT; ++c) { k.seen[c] = 0; k.timed[c] = 0; k.ms[c] = 0.0; k.bytes[c] = 0.0; }
+    return 0;
+}
+
+int tsr_solver_ktimer_read(tsr_solver* s, long long* seen, long long* timed, double* ms, double* bytes)
+{
+    if (hipStreamSynchronize(s->st) == hipSuccess && !s->kt.pend.empty()) kt_harvest(s->kt);
+    for (int c = 0; c < KT_COUNT; ++c) {
+        if (seen) seen[c] = s->kt.seen[c];
+        if (timed) timed[c] = s->kt.timed[c];
+        if (ms) ms[c] = s->kt.ms[c];
+        if (bytes) bytes[c] = s->kt.bytes[c];
+    }
+    return 0;
+}
+
+int tsr_solver_set_step(tsr_solver* s, double tau)
+{
+    if (!(tau > 0.0) || !std::isfinite(tau)) return 1;
+    for (auto& c : s->sc) { c.proposal = tau; c.stats.CurrentStep = tau; }
+    return 0;
+}
+
 int tsr_solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
 {
+    KtScope kts(s);
     return solver_step(s, tend, rec);
 }
 
 int tsr_solver_run(tsr_solver* s, double tend, int max_steps, tsr_run_stats* out, tsr_step_record* records,
                    int max_records)
 {
+    KtScope kts(s);
     const auto t0 = std::chrono::steady_clock::now();
     int steps = 0, rej = 0, fe = 0, rc = 0;
     s->t_dense = 0.0;
@@ -1193,7 +1298,7 @@ int tsr_solver_run(tsr_solver* s, double tend, int max_steps, tsr_run_stats* out
         if (records && steps * s->S < max_records)
             for (int q = 0; q < s->S && steps * s->S + q < max_records; ++q) records[steps * s->S + q] = rec[q];
     }
-    hipStreamSynchronize(s->st);
+    if (hipStreamSynchronize(s->st) == hipSuccess && !s->kt.pend.empty()) kt_harvest(s->kt);
     if (out) {
         out->t = s->sc[0].t;
         for (auto& c : s->sc) out->t = std::min(out->t, c.t);
@@ -1312,6 +1417,16 @@ int tsr_d2h(double* dst, const double* src, long long n)
 }
 
 // ---------------------------------------------------------------- reference signatures
+}  // extern "C"
+
+// The reference's RHS names (include/main.h:21-22), C++ linkage (refabi.cpp): both are
+// the device RHS, so exprem keeps the step on the device for them as for Gfunc_hip.
+int Gfunc_serial(double* g, double* u, user_data_struct user_data);
+int Gfunc_parallel(double* g, double* u, user_data_struct user_data);
+
+static bool device_rhs(tsr_gfunc_t G) { return G == Gfunc_hip || G == Gfunc_parallel || G == Gfunc_serial; }
+
+extern "C" {
 // field by field: the struct has padding, whose bytes a caller's copy need not repeat
 static bool same_options(const exprem_options_struct& a, const exprem_options_struct& b)
 {
@@ -1362,7 +1477,7 @@ int exprem_c(tsr_gfunc_t Gfunc, int N, double* t, double tend, double* tspan, do
     c.stats = *stats;
     c.proposal = stats->CurrentStep;
     c.t = *t;
-    s->host_g = (Gfunc == Gfunc_hip) ? nullptr : Gfunc;
+    s->host_g = device_rhs(Gfunc) ? nullptr : Gfunc;
     if (s->host_g) {
         s->host_ud = ud;
         s->hx.resize(N); s->hg.resize(N); s->hg0.resize(N);
