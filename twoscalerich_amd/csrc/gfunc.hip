@@ -354,6 +354,10 @@ __device__ __forceinline__ bool is_dirichlet(const DTmpl& T, int i)
     }
 }
 
+// Element soil of a micro cell from its soil words (staged in LDS): bit e set = soil A.
+constexpr int kMaxSoilWords = 64;   // micro templates of at most 2048 elements
+__device__ __forceinline__ bool soil_is_a(const unsigned* sw, int e) { return (sw[e >> 5] >> (e & 31)) & 1u; }
+
 // Net inflow of interior node i = (ix, iy) of the uniform right-triangle grid with
 // spacing d (1 <= ix, iy <= r-2: all six incident elements exist).  On this grid the
 // element geometry is exact rationals times d (checked against the template's own
@@ -391,9 +395,9 @@ __device__ __forceinline__ double q_struct(double h0, double h1, double h2, doub
 
 template <int TYPE, int LV>
 __device__ __forceinline__ void pair_struct(double& net, int e, int v0, int v1, int v2, const double* h,
-                                            const double* KA, const double* KB, const unsigned char* soil, double d)
+                                            const double* KA, const double* KB, const unsigned* sw, double d)
 {
-    const double* K = soil[e] == TSR_SOIL_A ? KA : KB;
+    const double* K = soil_is_a(sw, e) ? KA : KB;
     const double h0 = h[v0], h1 = h[v1], h2 = h[v2];
     const double k0 = K[v0], k1 = K[v1], k2 = K[v2];
     constexpr int JIN = LV == 0 ? 2 : LV - 1;
@@ -403,7 +407,7 @@ __device__ __forceinline__ void pair_struct(double& net, int e, int v0, int v1, 
 }
 
 __device__ __forceinline__ double struct_net(const DTmpl& T, int i, const double* h, const double* KA,
-                                             const double* KB, const unsigned char* soil)
+                                             const double* KB, const unsigned* sw)
 {
     const int r = T.r, rm = r - 1;
     const int iy = div_r(T, i), ix = i - iy * r;
@@ -412,12 +416,12 @@ __device__ __forceinline__ double struct_net(const DTmpl& T, int i, const double
     const int eLD = 2 * ((iy - 1) * rm + ix - 1), eRD = 2 * ((iy - 1) * rm + ix);
     const int eLU = 2 * (iy * rm + ix - 1), eRU = 2 * (iy * rm + ix);
     double net = 0.0;
-    pair_struct<0, 2>(net, eLD, iLD, iD, i, h, KA, KB, soil, d);
-    pair_struct<1, 1>(net, eLD + 1, iLD, i, iL, h, KA, KB, soil, d);
-    pair_struct<1, 2>(net, eRD + 1, iD, iR, i, h, KA, KB, soil, d);
-    pair_struct<0, 1>(net, eLU, iL, i, iU, h, KA, KB, soil, d);
-    pair_struct<0, 0>(net, eRU, i, iR, iRU, h, KA, KB, soil, d);
-    pair_struct<1, 0>(net, eRU + 1, i, iRU, iU, h, KA, KB, soil, d);
+    pair_struct<0, 2>(net, eLD, iLD, iD, i, h, KA, KB, sw, d);
+    pair_struct<1, 1>(net, eLD + 1, iLD, i, iL, h, KA, KB, sw, d);
+    pair_struct<1, 2>(net, eRD + 1, iD, iR, i, h, KA, KB, sw, d);
+    pair_struct<0, 1>(net, eLU, iL, i, iU, h, KA, KB, sw, d);
+    pair_struct<0, 0>(net, eRU, i, iR, iRU, h, KA, KB, sw, d);
+    pair_struct<1, 0>(net, eRU + 1, i, iRU, iU, h, KA, KB, sw, d);
     return net;
 }
 
@@ -425,7 +429,7 @@ __device__ __forceinline__ double struct_net(const DTmpl& T, int i, const double
 // pairs from the CSR in ascending element order, the two sub-face fluxes each pair
 // needs, net += q_in; net -= q_out.
 __device__ __forceinline__ double generic_net(const DTmpl& T, int i, const double* h, const double* KA,
-                                              const double* KB, const unsigned char* soil)
+                                              const double* KB, const unsigned* sw)
 {
     const int ne = T.ne;
     double net = 0.0;
@@ -435,7 +439,7 @@ __device__ __forceinline__ double generic_net(const DTmpl& T, int i, const doubl
         const double h0 = h[v0], h1 = h[v1], h2 = h[v2];
         const double gx = T.sf[e] * h0 + T.sf[ne + e] * h1 + T.sf[2 * ne + e] * h2;
         const double gy = T.sf[3 * ne + e] * h0 + T.sf[4 * ne + e] * h1 + T.sf[5 * ne + e] * h2;
-        const double* Ks = (soil[e] == TSR_SOIL_A) ? KA : KB;
+        const double* Ks = soil_is_a(sw, e) ? KA : KB;
         const double k0 = Ks[v0], k1 = Ks[v1], k2 = Ks[v2];
         const int jin = lv == 0 ? 2 : lv - 1;   // (lv + 2) % 3: the sub-face entering v_lv
         double qj[2];
@@ -470,8 +474,10 @@ __device__ __forceinline__ void lin_load(const DTmpl& T, const GArgs& a, size_t 
 {
     if (i >= nn) return;
     const long long gi = off + i;
-    x.g0 = a.g0[gi];
+    // G(u) is 0 at micro Dirichlet nodes, so the product is too: nothing to load there
+    x.g0 = 0.0;
     if (is_dirichlet<STRUCT>(T, i)) return;
+    x.g0 = a.g0[gi];
     x.u = a.u[gi];
     x.v = a.v ? a.v[gi] : 0.0;
     const double* L = a.lin + gi;
@@ -483,16 +489,17 @@ __device__ __forceinline__ void lin_load(const DTmpl& T, const GArgs& a, size_t 
     }
 }
 
-// Recorded head and the four conductivity components at a Dirichlet node.
+// Recorded head (element-hosted cells only: node-hosted cells take the head's offset
+// from the macro perturbation) and the four conductivity components at a Dirichlet node.
 struct LinDir {
     double hb, c[4];
 };
 
-__device__ __forceinline__ void lin_load_dir(const GArgs& a, size_t NN, long long gi, LinDir& x)
+__device__ __forceinline__ void lin_load_dir(const GArgs& a, size_t NN, long long gi, bool need_hb, LinDir& x)
 {
     const double* L = a.lin + gi;
     const float* Ld = reinterpret_cast<const float*>(a.lin + kLinComp64 * NN) + gi;
-    x.hb = L[0];
+    x.hb = need_hb ? L[0] : 0.0;
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
         x.c[2 * c] = L[(c + 2) * NN];
@@ -520,9 +527,12 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
 {
     extern __shared__ __align__(16) double smem[];
     __shared__ double s_bc[6];
-    __shared__ double s_mel[kMaxNodeInc][12];
+    // per incident macro element of a node-hosted cell: shape-function coefficients (9),
+    // vertex heads (3) and, for MODE 2, the vertex heads' perturbations (3)
+    __shared__ double s_mel[kMaxNodeInc][16];
     __shared__ double s_part[256];
     __shared__ double s_geo[2][22];
+    __shared__ unsigned s_sw[kMaxSoilWords];
     // the leading workgroups of the launch evaluate the macro elements (one thread each,
     // every scenario); the rest are the micro cells
     const int mwg = macro_wgs(B, blockDim.x);
@@ -545,7 +555,7 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
     // MODE 0/1: node storage capacity, then the node's net inflow.  MODE 2 keeps both in
     // registers (each thread owns its nodes from load to gather) and has no such array.
     double* stl = KB + B.nn_max;
-    unsigned char* soil = reinterpret_cast<unsigned char*>(MODE == 2 ? stl : stl + B.nn_max);
+    const unsigned* sw = s_sw;
     const double ve = a.v ? a.veps[s] : 0.0;
     const long long off = B.cell_off[cell];
     const int host = cell - S.first_cell;
@@ -553,85 +563,133 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
     const bool node_host = (S.host == TSR_HOST_NODE);
     const size_t NN = (size_t)B.N;
     const int nbnd = STRUCT ? 4 * (T.r - 1) : T.nbnd;
+    const int nsw = B.soil_words;
 
-    // MODE 2: this thread's first two nodes and its first Dirichlet node are loaded
-    // before the set-up phase below, so those loads are in flight during it
-    LinNode xn[2];
-    LinDir xd;
+    // ---- Loads.  Everything that depends only on the cell is issued before any of it is
+    // used, so one memory round trip (two for the macro heads around a node-hosted cell:
+    // vertex index, then head) precedes the first barrier: the cell's soil words; this
+    // thread's first two nodes (their state; MODE 2: the recorded expansion, and the
+    // recorded conductivities of the thread's first Dirichlet node); for node-hosted
+    // cells the host's head and position and, for models 2/3, the shape-function
+    // coefficients, vertex heads and vertex perturbations of the host's incident macro
+    // elements (unused slots hold zero coefficients and the Dirichlet head).
+    const unsigned swv = tid < nsw ? B.soil_bits[(size_t)cell * nsw + tid] : 0u;
     int best = 0;   // node-hosted cells: macro element of this thread's first Dirichlet node
     if (node_host && model != 1 && tid < nbnd) best = B.dir_elem[(size_t)cell * B.nn_max + dirichlet_node<STRUCT>(T, tid)];
+    LinNode xn[2];
+    LinDir xd;
+    double hs[2] = {0.0, 0.0};   // MODE 0/1: state at this thread's first two nodes
     if constexpr (MODE == 2) {
         lin_load<STRUCT>(T, a, NN, off, nn, tid, xn[0]);
         lin_load<STRUCT>(T, a, NN, off, nn, tid + bs, xn[1]);
-        if (tid < nbnd) lin_load_dir(a, NN, off + dirichlet_node<STRUCT>(T, tid), xd);
+        if (tid < nbnd) lin_load_dir(a, NN, off + dirichlet_node<STRUCT>(T, tid), !node_host, xd);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int i = tid + k * bs;
+            if (i < nn && !is_dirichlet<STRUCT>(T, i)) hs[k] = ld_state(a, ve, off + i);
+        }
+    }
+    const bool has_mel = node_host && model != 1 && tid < kMaxNodeInc;
+    double mel[15];
+    if (has_mel) {
+        const double* sf = B.cm_sf + ((size_t)cell * kMaxNodeInc + tid) * 9;
+        const int* vi = B.cm_vi + ((size_t)cell * kMaxNodeInc + tid) * 3;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) mel[k] = sf[k];
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+            const int gi = vi[v];
+            mel[9 + v] = state_or(a, ve, gi, S.dirichlet_head);
+            mel[12 + v] = (MODE == 2 && a.v && gi >= 0) ? ve * a.v[gi] : 0.0;
+        }
+    }
+    double hb0 = 0.0, hb1 = 0.0;   // tid 0 of a node-hosted cell: host head and its perturbation
+    if (node_host && tid == 0) {
+        // the host's state and its Dirichlet flag are loaded side by side
+        const long long gh = S.u_off + host;
+        const double hu = ld_state(a, ve, gh);
+        const double pu = (MODE == 2 && a.v) ? ve * a.v[gh] : 0.0;
+        const bool hdir = S.dir[host] != 0;
+        hb0 = hdir ? S.dirichlet_head : hu;
+        hb1 = hdir ? 0.0 : pu;
     }
 
-    // host interpolation data
-    if (!node_host) {
-        if (tid == 0) {
+    // ---- LDS stores of the loaded data (the element-hosted interpolant is formed here)
+    if (tid < nsw) s_sw[tid] = swv;
+    if (has_mel) {
+#pragma unroll
+        for (int k = 0; k < 15; ++k) s_mel[tid][k] = mel[k];
+    }
+    if (tid == 0) {
+        if (!node_host) {
             double hv[4], al[4];
             macro_interp(S, a, ve, host, hv, al);
             if (S.nv == 3) { s_bc[0] = al[0]; s_bc[1] = al[1]; s_bc[2] = 0.0; s_bc[3] = al[2]; }
             else { s_bc[0] = al[0]; s_bc[1] = al[1]; s_bc[2] = al[2]; s_bc[3] = al[3]; }
             s_bc[4] = S.cen[host];
             s_bc[5] = S.cen[S.ne + host];
-        }
-    } else {
-        const int ninc = B.cm_ninc[cell];
-        if (tid == 0) {
-            s_bc[3] = state_or(a, ve, B.cm_host[cell], S.dirichlet_head);
+        } else {
+            s_bc[2] = hb1;   // host perturbation (MODE 2, model 1)
+            s_bc[3] = hb0;
             s_bc[4] = S.nodes[host];
             s_bc[5] = S.nodes[S.nn + host];
-        }
-        if (model != 1 && tid < ninc) {
-            const double* sf = B.cm_sf + ((size_t)cell * kMaxNodeInc + tid) * 9;
-            const int* vi = B.cm_vi + ((size_t)cell * kMaxNodeInc + tid) * 3;
-#pragma unroll
-            for (int k = 0; k < 9; ++k) s_mel[tid][k] = sf[k];
-#pragma unroll
-            for (int v = 0; v < 3; ++v) s_mel[tid][9 + v] = state_or(a, ve, vi[v], S.dirichlet_head);
         }
     }
     if constexpr (STRUCT) {
         if (tid < 44) s_geo[tid / 22][tid % 22] = T.sg[tid / 22][tid % 22];
     }
-    __syncthreads();
-
-    // MODE 2: interior heads, storage capacities (registers) and conductivities (LDS)
-    // from the recorded expansion, then the Dirichlet nodes below with their macro-field
-    // heads; everything is ready after one barrier.
+    // MODE 2: interior heads, storage capacities (registers) and conductivities (LDS) of
+    // this thread's first two nodes from the recorded expansion; MODE 0/1: their heads
     double g0r[kLinNPT], str[kLinNPT];   // MODE 2: g0 and storage of this thread's nodes
+    auto lin_node = [&](int k, const LinNode& x) {
+        const int i = tid + k * bs;
+        g0r[k] = 0.0;
+        str[k] = 1.0;
+        if (i >= nn) return;
+        g0r[k] = x.g0;
+        if (is_dirichlet<STRUCT>(T, i)) return;
+        // the perturbation itself is the offset from the expansion point (the recorded
+        // head is u), so component 0 is not read here
+        const double dv = a.v ? ve * x.v : 0.0;
+        h[i] = x.u + dv;
+        str[k] = x.c[0] + x.c[1] * dv;
+        KA[i] = x.c[2] + x.c[3] * dv;
+        KB[i] = x.c[4] + x.c[5] * dv;
+    };
     if constexpr (MODE == 2) {
+        lin_node(0, xn[0]);
+        lin_node(1, xn[1]);
+    } else {
 #pragma unroll
-        for (int k = 0; k < kLinNPT; ++k) {
+        for (int k = 0; k < 2; ++k) {
             const int i = tid + k * bs;
-            g0r[k] = 0.0;
-            str[k] = 1.0;
-            if (i >= nn) continue;
-            // nodes 0 and 1 were loaded before the set-up phase; nodes 2 and 3 (templates
-            // with more than two nodes per thread) are loaded here
-            LinNode& x = xn[k & 1];
-            if (k >= 2) lin_load<STRUCT>(T, a, NN, off, nn, i, x);
-            g0r[k] = x.g0;
-            if (is_dirichlet<STRUCT>(T, i)) continue;
-            // interior nodes: the perturbation itself is the offset from the expansion
-            // point (the recorded head is u), so component 0 is not read here
-            const double dv = a.v ? ve * x.v : 0.0;
-            const double hi = x.u + dv;
-            h[i] = hi;
-            str[k] = x.c[0] + x.c[1] * dv;
-            KA[i] = x.c[2] + x.c[3] * dv;
-            KB[i] = x.c[4] + x.c[5] * dv;
+            if (i < nn && !is_dirichlet<STRUCT>(T, i)) h[i] = hs[k];
         }
     }
+    __syncthreads();
 
-    // heads: Dirichlet nodes from the macro field (first nbnd threads), the rest from u.
+    // ---- the rest of the nodes (templates with more than two nodes per thread)
+    if constexpr (MODE == 2) {
+#pragma unroll
+        for (int k = 2; k < kLinNPT; ++k) {
+            LinNode& x = xn[k & 1];
+            lin_load<STRUCT>(T, a, NN, off, nn, tid + k * bs, x);
+            lin_node(k, x);
+        }
+    } else {
+        for (int i = tid + 2 * bs; i < nn; i += bs)
+            if (!is_dirichlet<STRUCT>(T, i)) h[i] = ld_state(a, ve, off + i);
+    }
+    // Dirichlet heads from the macro field (first nbnd threads); MODE 2 also expands the
+    // conductivities there about the recorded head, by the head's perturbation d
     for (int b = tid; b < nbnd; b += bs) {
         const int i = dirichlet_node<STRUCT>(T, b);
-        double hi;
+        double hi, d = 0.0;
         if (node_host) {
             if (model == 1) {
                 hi = s_bc[3];
+                d = s_bc[2];
             } else {
                 // P1 interpolation in the incident macro element chosen at upload
                 // (k_dir_elem): weights max(lambda, 0), normalised
@@ -639,14 +697,16 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
                 dir_point(S, T, s_bc[4], s_bc[5], i, px, py);
                 if (b != tid) best = B.dir_elem[(size_t)cell * B.nn_max + i];
                 macro_bary(s_mel[best], px, py, lam);
-                double ws = 0.0, hs = 0.0;
+                double ws = 0.0, hsum = 0.0, psum = 0.0;
 #pragma unroll
                 for (int v = 0; v < 3; ++v) {
                     const double w = fmax(lam[v], 0.0);
                     ws += w;
-                    hs += w * s_mel[best][9 + v];
+                    hsum += w * s_mel[best][9 + v];
+                    psum += w * s_mel[best][12 + v];
                 }
-                hi = hs / ws;
+                hi = hsum / ws;
+                d = psum / ws;
             }
         } else {
             const double X = (model == 1) ? s_bc[4] : s_bc[4] + T.vec[i];
@@ -655,20 +715,11 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
         }
         h[i] = hi;
         if constexpr (MODE == 2) {
-            if (b != tid) lin_load_dir(a, NN, off + i, xd);
-            const double d = hi - xd.hb;
+            if (b != tid) lin_load_dir(a, NN, off + i, !node_host, xd);
+            if (!node_host) d = hi - xd.hb;
             KA[i] = xd.c[0] + xd.c[1] * d;
             KB[i] = xd.c[2] + xd.c[3] * d;
         }
-    }
-    if constexpr (MODE != 2) {
-        for (int i = tid; i < nn; i += bs)
-            if (!is_dirichlet<STRUCT>(T, i)) h[i] = ld_state(a, ve, off + i);
-    }
-    {
-        const unsigned* sb = B.soil_bits + (size_t)cell * B.soil_words;
-#pragma unroll 1
-        for (int e = tid; e < ne; e += bs) soil[e] = ((sb[e >> 5] >> (e & 31)) & 1u) ? TSR_SOIL_A : TSR_SOIL_B;
     }
     __syncthreads();
 
@@ -679,7 +730,7 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
                 double cva = 0.0, cvb = 0.0;
                 for_incident<STRUCT>(T, i, [&](int e, int) {
                     const double sc = STRUCT ? s_geo[e & 1][21] : T.scv[e];
-                    if (soil[e] == TSR_SOIL_A) cva += sc; else cvb += sc;
+                    if (soil_is_a(sw, e)) cva += sc; else cvb += sc;
                 });
                 double cA = 0.0, cB = 0.0, kA = 0.0, kB = 0.0, dcA = 0.0, dcB = 0.0, dkA = 0.0, dkB = 0.0;
                 // first evaluation with the lane's own soil (no divergence), second only at interfaces
@@ -726,8 +777,8 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
     // v_{j+1}): net = sum q_{(lv+2)%3} - q_{lv}.  Each sub-face flux is the same
     // expression whichever node evaluates it, so the result does not depend on who does.
     auto node_net = [&](int i) -> double {
-        if constexpr (STRUCT) return struct_net(T, i, h, KA, KB, soil);
-        else return generic_net(T, i, h, KA, KB, soil);
+        if constexpr (STRUCT) return struct_net(T, i, h, KA, KB, sw);
+        else return generic_net(T, i, h, KA, KB, sw);
     };
     if constexpr (MODE == 2) {
         // the exchange slots (node i -> slot i % 256) this thread owns: with bs dividing
@@ -1071,6 +1122,10 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
     const size_t o_Q = A.reserve<double>((size_t)v.n_cells);
     const size_t o_de = A.reserve<unsigned char>((size_t)v.n_cells * nn_max);
     const int soil_words = (ne_max + 31) / 32;
+    if (soil_words > kMaxSoilWords) {
+        err = "micro templates of more than " + std::to_string(32 * kMaxSoilWords) + " elements are not supported";
+        return nullptr;
+    }
     const size_t o_sb = A.reserve<unsigned>((size_t)v.n_cells * soil_words);
     const size_t o_cmsf = A.reserve<double>((size_t)v.n_cells * kMaxNodeInc * 9);
     const size_t o_cmvi = A.reserve<int>((size_t)v.n_cells * kMaxNodeInc * 3);
@@ -1168,10 +1223,8 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
     }
     P->all_structured = !std::getenv("TSR_GENERIC_MICRO");
     for (int t = 0; t < v.n_tmpl; ++t) P->all_structured = P->all_structured && to[t].structured;
-    P->lds = (size_t)4 * nn_max * sizeof(double) + (size_t)ne_max;
-    P->lds = (P->lds + 15) & ~size_t(15);
-    P->lds2 = (size_t)3 * nn_max * sizeof(double) + (size_t)ne_max;   // linearised mode: no storage array
-    P->lds2 = (P->lds2 + 15) & ~size_t(15);
+    P->lds = (size_t)4 * nn_max * sizeof(double);
+    P->lds2 = (size_t)3 * nn_max * sizeof(double);   // linearised mode: no storage array
     if (P->lds > 64 * 1024) {
         bool ok = true;
         for (const void* f : micro_kernels()) ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds) == hipSuccess;
