@@ -269,6 +269,18 @@ int tsr_solver_set_path(tsr_solver* s, int path);
  * (lin = 0) or the TSR_PATH_JVLIN expansion (lin = 1); v, jv host arrays of N. */
 int tsr_solver_jv(tsr_solver* s, const double* v, double eps, int lin, double* jv);
 
+/* tsr_solver_jv without any device allocation, for callers that cannot allocate (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Same arguments and result as tsr_solver_jv; the
+ * per-scenario FD step lives in a static device array of 2 x 4096 doubles (one per GPU,
+ * kept until the process ends, shared by every call on that GPU: calls that overlap on
+ * one GPU race), so the problem may hold at most 4096 scenarios.  lin = 1 needs the
+ * linearisation record allocated beforehand (tsr_solver_set_path with TSR_PATH_JVLIN).
+ * A micro-cell kernel not tuned yet runs with a fixed launch shape.  Returns 0, 1 on
+ * a rejected argument (too many scenarios, no record, eps zero or not finite), 2 on a
+ * device error. */
+int tsr_solver_jv_noalloc(tsr_solver* s, const double* v, double eps, int lin, double* jv);
+
 /* Raw device pointers (for tests and the bench): u, a scratch vector of length N. */
 double* tsr_solver_dev_u(tsr_solver* s);
 double* tsr_solver_dev_scratch(tsr_solver* s, int which);

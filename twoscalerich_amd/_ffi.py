@@ -149,6 +149,7 @@ def declare(lib):
         "tsr_solver_set_path": (C.c_int, [P, C.c_int]),
         "tsr_solver_set_seed": (None, [P, C.c_ulonglong]),
         "tsr_solver_jv": (C.c_int, [P, _dp, C.c_double, C.c_int, _dp]),
+        "tsr_solver_jv_noalloc": (C.c_int, [P, _dp, C.c_double, C.c_int, _dp]),
         "tsr_solver_set_mlp": (C.c_int, [P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "tsr_mlp_param_count": (C.c_int, []),
         "tsr_mlp_create": (P, []),

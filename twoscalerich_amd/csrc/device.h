@@ -144,9 +144,11 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err);
 void        free_problem(DevProblem* p);
 const DBatch& batch_of(const DevProblem* p);
 size_t      micro_lds_bytes(const DevProblem* p);
-hipError_t  launch_gfunc(const DevProblem* p, const GArgs& a, hipStream_t st);
+// tune = false: a micro-cell kernel not tuned yet runs with a fixed admissible shape
+// instead of timing the candidates (no events are created)
+hipError_t  launch_gfunc(const DevProblem* p, const GArgs& a, hipStream_t st, bool tune = true);
 // one of the three G kernels: 0 macro elements, 1 micro cells, 2 macro nodes
-hipError_t  launch_gfunc_part(const DevProblem* p, const GArgs& a, int part, hipStream_t st);
+hipError_t  launch_gfunc_part(const DevProblem* p, const GArgs& a, int part, hipStream_t st, bool tune = true);
 // the linearised mode needs micro templates of at most 1024 nodes
 bool        lin_mode_ok(const DevProblem* p);
 // tuned micro-cell launch shape (bs = 0 before the first G)
@@ -232,6 +234,15 @@ hipError_t launch_dcgs2_f32(int nk, const float* const* cols32, const double* Rj
 hipError_t launch_to_f32(float* dst, const double* src, const BlkMap& M, int nblk, const int* mask, hipStream_t st);
 hipError_t launch_masked_copy(double* dst, const double* src, const BlkMap& M, int nblk, const int* mask,
                               hipStream_t st);
+
+// Allocation-free FD products (tsr_solver_jv_noalloc): the per-scenario FD step and its
+// inverse, [eps x S | 1/eps x S], live in a file-scope device array of
+// 2 * kNoAllocMaxScen doubles (krylov.hip).  noalloc_scratch() gives the calling thread's
+// current device's address of it (resolved once per device), or nullptr on error;
+// launch_fill_eps writes the 2 S entries.
+constexpr int kNoAllocMaxScen = 4096;
+double*    noalloc_scratch();
+hipError_t launch_fill_eps(double* de, int n_scen, double eps, hipStream_t st);
 // Readbacks into a mapped pinned host buffer (layouts in krylov.hip); both clear the
 // bad flags they read
 hipError_t launch_pack_window(const KrylovState& K, int c0, int jt, int* bad, const int* active, int n_scen,

@@ -1287,18 +1287,25 @@ static void launch_micro(const DevProblem* p, const GArgs& a, int bs, int wpe, h
 // launch shape on the real inputs (one warm-up and three timed launches each, HIP
 // events) and keep the fastest.  Every candidate computes the same result, so the
 // output of the final launch (the chosen shape) is the G the caller asked for.
-static hipError_t tune_micro(DevProblem* p, const GArgs& a, hipStream_t st)
+// Workgroup sizes the micro-cell kernel admits for a mode: at most 8 strided nodes per
+// thread; the linearised mode also needs bs to divide 256 and at most kLinNPT nodes per
+// thread.
+static std::vector<int> micro_bs_candidates(const DBatch& B, int mode)
 {
-    const DBatch& B = p->B;
-    const int mode = micro_mode(a);
     std::vector<int> cand_bs;
     for (int bs : {64, 128, 192, 256, 384, 512, 768, 1024}) {
-        // at most 8 strided nodes per thread; the linearised mode also needs bs to divide
-        // 256 and at most kLinNPT nodes per thread
         const bool ok2 = mode != 2 || (256 % bs == 0 && bs * kLinNPT >= B.nn_max);
         if (bs <= kMicroMaxBS && bs * 8 >= B.nn_max && ok2) cand_bs.push_back(bs);
     }
     if (cand_bs.empty()) cand_bs.push_back(kMicroMaxBS);
+    return cand_bs;
+}
+
+static hipError_t tune_micro(DevProblem* p, const GArgs& a, hipStream_t st)
+{
+    const DBatch& B = p->B;
+    const int mode = micro_mode(a);
+    const std::vector<int> cand_bs = micro_bs_candidates(B, mode);
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return hipErrorUnknown;
     // register budgets whose code spills to scratch are not candidates: besides being
@@ -1338,7 +1345,7 @@ static hipError_t tune_micro(DevProblem* p, const GArgs& a, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_gfunc_part(const DevProblem* p, const GArgs& a, int part, hipStream_t st)
+hipError_t launch_gfunc_part(const DevProblem* p, const GArgs& a, int part, hipStream_t st, bool tune)
 {
     const DBatch& B = p->B;
     if (part == 0) {
@@ -1346,7 +1353,14 @@ hipError_t launch_gfunc_part(const DevProblem* p, const GArgs& a, int part, hipS
     } else if (part == 1) {
         const int mode = micro_mode(a);
         if (mode == 2 && !lin_mode_ok(p)) return hipErrorInvalidValue;
-        if (p->micro_bs[mode] == 0) return tune_micro(const_cast<DevProblem*>(p), a, st);
+        if (p->micro_bs[mode] == 0) {
+            if (tune) return tune_micro(const_cast<DevProblem*>(p), a, st);
+            // not tuned yet and tuning not wanted (it creates events): the smallest
+            // admissible shape with the compiler's register budget (results do not
+            // depend on the shape)
+            launch_micro(p, a, micro_bs_candidates(p->B, mode)[0], 1, st);
+            return hipGetLastError();
+        }
         // algorithmic bytes: u (and v, G(u) for an FD product) read, the result written,
         // per micro node
         const double bpu = 8.0 * (2 + (a.v ? 1 : 0) + (a.g0 ? 1 : 0));
@@ -1371,10 +1385,10 @@ void micro_shape(const DevProblem* p, int* bs, int* wpe)
     *wpe = p->micro_wpe[0];
 }
 
-hipError_t launch_gfunc(const DevProblem* p, const GArgs& a, hipStream_t st)
+hipError_t launch_gfunc(const DevProblem* p, const GArgs& a, hipStream_t st, bool tune)
 {
     for (int part = 0; part < 3; ++part) {
-        const hipError_t e = launch_gfunc_part(p, a, part, st);
+        const hipError_t e = launch_gfunc_part(p, a, part, st, tune);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

@@ -17,6 +17,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <mutex>
 
 namespace tsr {
 
@@ -848,6 +849,42 @@ hipError_t launch_dcgs2_f32(int nk, const float* const* cols32, const double* Rj
 hipError_t launch_to_f32(float* dst, const double* src, const BlkMap& M, int nblk, const int* mask, hipStream_t st)
 {
     hipLaunchKernelGGL(k_to_f32, dim3(nblk), dim3(kVecBS), 0, st, dst, src, M, mask);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- allocation-free scratch
+// The FD steps of tsr_solver_jv_noalloc: one array per GPU, held until the process
+// ends, shared by every call on that GPU (calls that overlap on one GPU race).  Host
+// code reaches it only through the address hipGetSymbolAddress gives for each device.
+static __device__ __align__(256) double scratch[2 * kNoAllocMaxScen];
+
+double* noalloc_scratch()
+{
+    constexpr int kMaxDev = 64;
+    static std::mutex mu;
+    static double* addr[kMaxDev] = {};
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+        addr[dev] = static_cast<double*>(p);
+    }
+    return addr[dev];
+}
+
+__global__ void k_fill_eps(double* de, int n_scen, double eps)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_scen) return;
+    de[q] = eps;
+    de[n_scen + q] = 1.0 / eps;
+}
+
+hipError_t launch_fill_eps(double* de, int n_scen, double eps, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill_eps, dim3((n_scen + 255) / 256), dim3(256), 0, st, de, n_scen, eps);
     return hipGetLastError();
 }
 

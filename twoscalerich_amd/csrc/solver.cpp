@@ -1337,6 +1337,36 @@ int tsr_solver_jv(tsr_solver* s, const double* v_host, double eps, int lin, doub
     return 0;
 }
 
+int tsr_solver_jv_noalloc(tsr_solver* s, const double* v_host, double eps, int lin, double* jv_host)
+{
+    const int fail_code = 2;
+    if (s->S > kNoAllocMaxScen) {
+        set_error("tsr_solver_jv_noalloc: at most " + std::to_string(kNoAllocMaxScen) + " scenarios");
+        return 1;
+    }
+    if (lin && !s->linrec) {
+        set_error("tsr_solver_jv_noalloc: the linearisation record is not allocated (set TSR_PATH_JVLIN first)");
+        return 1;
+    }
+    if (!(eps != 0.0) || !std::isfinite(eps)) { set_error("tsr_solver_jv_noalloc: eps must be finite and nonzero"); return 1; }
+    double* de = noalloc_scratch();
+    if (!de) { set_error("tsr_solver_jv_noalloc: device scratch address unavailable"); return 2; }
+    const size_t N = (size_t)s->N;
+    HIPCHK(launch_fill_eps(de, s->S, eps, s->st));
+    HIPCHK(hipMemcpyAsync(s->scratch[0], v_host, sizeof(double) * N, hipMemcpyHostToDevice, s->st));
+    GArgs a;
+    a.u = s->u; a.v = nullptr; a.veps = nullptr; a.g0 = nullptr; a.ieps = nullptr; a.active = nullptr; a.out = s->g0;
+    a.rec = lin ? s->linrec : nullptr;
+    HIPCHK(launch_gfunc(s->P, a, s->st, false));
+    a.v = s->scratch[0]; a.veps = de; a.g0 = s->g0; a.ieps = de + s->S; a.out = s->scratch[1];
+    a.rec = nullptr;
+    a.lin = lin ? s->linrec : nullptr;
+    HIPCHK(launch_gfunc(s->P, a, s->st, false));
+    HIPCHK(hipMemcpyAsync(jv_host, s->scratch[1], sizeof(double) * N, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return 0;
+}
+
 int tsr_solver_set_path(tsr_solver* s, int path)
 {
     if (path & ~(TSR_PATH_BATCHED | TSR_PATH_F32BASIS)) return 1;
