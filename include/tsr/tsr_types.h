@@ -15,6 +15,11 @@
 extern "C" {
 #endif
 
+/* A caller that already has the reference's own definitions of these seven structs (its
+ * main.h, user_data.h, ...: they carry no include guards) defines
+ * TSR_USE_REFERENCE_HEADERS before including this header, which then uses those. */
+#ifndef TSR_USE_REFERENCE_HEADERS
+
 /* include/macro_mesh.h:1-19 */
 struct macro_mesh_struct {
     int    no_verts;                 /* 3 = P1 triangles, 4 = Q1 quads */
@@ -115,6 +120,8 @@ struct exprem_stats_struct {
     int     NumSteps;
     int     FuncEvals;
 };
+
+#endif /* TSR_USE_REFERENCE_HEADERS */
 
 #ifndef __cplusplus
 typedef struct macro_mesh_struct     macro_mesh_struct;

@@ -7,8 +7,9 @@
 // coefficients of Hochbruck, Lubich & Selhofer (SISC 19(5) 1998, p.1567), LU solve, then
 // s doublings phi(2A) = (e^A + I) phi(A) / 2, e^{2A} = (e^A)^2.  The coefficient table
 // must equal the published one.  Differences from the reference: a singular
-// denominator returns 1 (the reference calls exit), and m = 1 with H = 0 returns 1
-// (the reference's (e^a - 1) a / log(e^a) / a is 0/0 there).
+// denominator, or a non-finite entry of alpha H, returns 1 (the reference calls exit on
+// the first and carries the second through), and m = 1 with H = 0 gives phi_1 = 1 (the
+// reference's (e^a - 1) a / log(e^a) / a is 0/0 there).
 #include "tsr_internal.h"
 
 #include <cmath>
@@ -150,6 +151,11 @@ TSR_HOST_CLONES bool solve(int n, double* A, double* B, int nrhs, double* R)
 
 int phi1_pade(double* phiH, const double* H, double alpha, int m, int ldh, int ldphi)
 {
+    // a non-finite entry leaves the Pade denominator undefined: reported as singular
+    // (the scaling exponent below would otherwise convert an infinite logarithm to int)
+    for (int j = 0; j < m; ++j)
+        for (int i = 0; i < m; ++i)
+            if (!std::isfinite(alpha * H[i + (size_t)j * ldh])) return 1;
     if (m == 1) {
         const double a = alpha * H[0];
         phiH[0] = (a == 0.0) ? 1.0 : std::expm1(a) / a;

@@ -1501,7 +1501,12 @@ int exprem_c(tsr_gfunc_t Gfunc, int N, double* t, double tend, double* tspan, do
              user_data_struct ud, exprem_options_struct options, exprem_stats_struct* stats)
 {
     tsr_solver* s = solver_for(ud, &options);
-    if (!s || s->N != N || s->S != 1) return -1;
+    if (!s) return -1;
+    if (s->N != N || s->S != 1) {
+        set_error("exprem: N = " + std::to_string(N) + " does not match the problem of user_data (N = " +
+                  std::to_string(s->N) + ", " + std::to_string(s->S) + " scenarios)");
+        return -1;
+    }
     if (*t == 0.0) exprem_stats_init(stats, &options, u0);
     ScState& c = s->sc[0];
     c.stats = *stats;
@@ -1512,7 +1517,10 @@ int exprem_c(tsr_gfunc_t Gfunc, int N, double* t, double tend, double* tspan, do
         s->host_ud = ud;
         s->hx.resize(N); s->hg.resize(N); s->hg0.resize(N);
     }
-    if (hipMemcpy(s->u, u0, sizeof(double) * N, hipMemcpyHostToDevice) != hipSuccess) return -1;
+    if (hipMemcpy(s->u, u0, sizeof(double) * N, hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("exprem: copying u0 to the device failed");
+        return -1;
+    }
     // land exactly on the next output time
     double tstop = tend;
     int out_k = -1;
@@ -1522,7 +1530,10 @@ int exprem_c(tsr_gfunc_t Gfunc, int N, double* t, double tend, double* tspan, do
     const int rc = solver_step(s, tstop, nullptr);
     s->host_g = nullptr;
     if (rc) return -1;
-    if (hipMemcpy(u0, s->u, sizeof(double) * N, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    if (hipMemcpy(u0, s->u, sizeof(double) * N, hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("exprem: copying the solution from the device failed");
+        return -1;
+    }
     *t = c.t;
     if (out_k > 0 && tspan_solns && c.t == tstop) std::memcpy(tspan_solns + (size_t)out_k * N, u0, sizeof(double) * N);
     *stats = c.stats;
@@ -1546,5 +1557,8 @@ int exprem(int (*Gfunc)(double* g, double* u, user_data_struct user_data), int N
            double* tspan, double* tspan_solns, double* u0, user_data_struct user_data, exprem_options_struct options,
            exprem_stats_struct& stats)
 {
-    return exprem_c(Gfunc, N, &t, tend, tspan, tspan_solns, u0, user_data, options, &stats);
+    // the reference's driver stops on -1 without a message (main.cpp:319-322): say why
+    const int rc = exprem_c(Gfunc, N, &t, tend, tspan, tspan_solns, u0, user_data, options, &stats);
+    if (rc) std::fprintf(stderr, "exprem: step failed at t = %.6e: %s\n", t, tsr_last_error());
+    return rc;
 }
