@@ -54,6 +54,15 @@ void check_step(double dt, double MinStep);
  * flag != 0, unchanged otherwise). */
 double check_flag_Gfunc(int flag, double dt);
 
+/* Frees the device copy and the cached solver that Gfunc_hip / Gfunc_* / exprem made for
+ * a caller's own user_data (the reference's driver never frees its meshes,
+ * main.cpp:357-361, so these live until this call or the process end).  A user_data
+ * whose arrays were edited in place or reallocated is noticed by a checksum on the next
+ * call and re-uploaded.  No counterpart in the reference.  Returns 0, or 1 when nothing
+ * was adopted for this user_data (problems of tsr_problem_* are freed by
+ * tsr_problem_free). */
+int tsr_release(user_data_struct user_data);
+
 /* Defaults of src/set_exprem_options.cpp:16-26 (C twin of set_exprem_options). */
 int set_exprem_options_c(exprem_options_struct* options);
 

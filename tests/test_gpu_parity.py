@@ -114,6 +114,36 @@ def test_gfunc_hip_unregistered_user_data():
     p.close()
 
 
+def test_gfunc_hip_notices_arrays_edited_in_place_and_releases():
+    """An adopted user_data whose arrays are edited in place (here the Keff table, scaled
+    by 2) is re-uploaded on the next call (a checksum of its arrays), and tsr_release
+    frees what was adopted for it."""
+    import ctypes as C
+    p = reference_problem(model=3)
+    ud = F.user_data_struct.from_buffer_copy(p.ud.contents)
+    M = ud.macro_mesh
+    el = np.ctypeslib.as_array(M.elements, shape=(M.no_verts * M.no_elements,)).copy()
+    ud.macro_mesh.elements = el.ctypes.data_as(C.POINTER(C.c_int))
+    u = seeded_state(p, 8, -3.0, -0.2)
+    L = T.lib()
+    g1 = np.empty_like(u)
+    g2 = np.empty_like(u)
+    assert L.Gfunc_hip(T._dp(g1), T._dp(u), ud) == 0, L.tsr_last_error()
+    K = ud.effective_conductivity
+    kv = np.ctypeslib.as_array(K.Keff_values, shape=(4 * K.no_hvalues,))
+    kv *= 2.0                                  # the same array, new contents
+    assert L.Gfunc_hip(T._dp(g2), T._dp(u), ud) == 0, L.tsr_last_error()
+    o = Oracle(p.view)                         # p's view shares the edited table
+    g_ref, _ = o.gfunc(u)
+    err = rel_inf(g2, g_ref)
+    print(f"edited Keff: rel_inf vs oracle {err:.3e}, changed entries {np.sum(g1 != g2)}", flush=True)
+    assert err <= G_TOL and not np.array_equal(g1, g2)
+    assert L.tsr_release(ud) == 0 and L.tsr_release(ud) == 1
+    kv /= 2.0
+    o.close()
+    p.close()
+
+
 def test_exprem_steps_match_oracle():
     """20 EXPREM steps of the plain small workload on the GPU and on the CPU oracle."""
     p = T.Problem.config(F.CFG_PLAIN_SMALL)

@@ -124,6 +124,7 @@ def declare(lib):
         "tsr_problem_options": (None, [P, C.POINTER(exprem_options_struct)]),
         "tsr_problem_distinct_bytes": (C.c_longlong, [P]),
         "tsr_last_error": (C.c_char_p, []),
+        "tsr_release": (C.c_int, [user_data_struct]),
         "tsr_statistics_write": (C.c_int, [C.c_char_p, C.POINTER(tsr_step_record), C.c_int, C.c_int, C.c_int]),
         "tsr_template_structured": (C.c_int, [C.POINTER(micro_mesh_struct)]),
         "tsr_solver_create": (P, [C.POINTER(tsr_batch_view), C.POINTER(exprem_options_struct)]),

@@ -465,7 +465,8 @@ constexpr int kLinNPT = 4;
 // ones.  Loading into this register block first and applying later lets a thread issue
 // every load of its nodes before it waits on any of them.
 struct LinNode {
-    double g0, u, v, c[6];
+    double u, v, c[3];   // state, direction, recorded storage / K_A / K_B (FP64)
+    float d[3];          // their recorded h-derivatives (FP32, widened where used)
 };
 
 template <bool STRUCT>
@@ -474,25 +475,23 @@ __device__ __forceinline__ void lin_load(const DTmpl& T, const GArgs& a, size_t 
 {
     if (i >= nn) return;
     const long long gi = off + i;
-    // G(u) is 0 at micro Dirichlet nodes, so the product is too: nothing to load there
-    x.g0 = 0.0;
     if (is_dirichlet<STRUCT>(T, i)) return;
-    x.g0 = a.g0[gi];
     x.u = a.u[gi];
     x.v = a.v ? a.v[gi] : 0.0;
     const double* L = a.lin + gi;
     const float* Ld = reinterpret_cast<const float*>(a.lin + kLinComp64 * NN) + gi;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        x.c[2 * c] = L[(c + 1) * NN];
-        x.c[2 * c + 1] = (double)Ld[c * NN];
+        x.c[c] = L[(c + 1) * NN];
+        x.d[c] = Ld[c * NN];
     }
 }
 
 // Recorded head (element-hosted cells only: node-hosted cells take the head's offset
 // from the macro perturbation) and the four conductivity components at a Dirichlet node.
 struct LinDir {
-    double hb, c[4];
+    double hb, c[2];     // recorded head, K_A, K_B (FP64)
+    float d[2];          // h-derivatives of K_A, K_B (FP32)
 };
 
 __device__ __forceinline__ void lin_load_dir(const GArgs& a, size_t NN, long long gi, bool need_hb, LinDir& x)
@@ -502,8 +501,8 @@ __device__ __forceinline__ void lin_load_dir(const GArgs& a, size_t NN, long lon
     x.hb = need_hb ? L[0] : 0.0;
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-        x.c[2 * c] = L[(c + 2) * NN];
-        x.c[2 * c + 1] = (double)Ld[(c + 1) * NN];
+        x.c[c] = L[(c + 2) * NN];
+        x.d[c] = Ld[(c + 1) * NN];
     }
 }
 
@@ -530,7 +529,7 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
     // per incident macro element of a node-hosted cell: shape-function coefficients (9),
     // vertex heads (3) and, for MODE 2, the vertex heads' perturbations (3)
     __shared__ double s_mel[kMaxNodeInc][16];
-    __shared__ double s_part[256];
+    __shared__ double s_q[4];
     __shared__ double s_geo[2][22];
     __shared__ unsigned s_sw[kMaxSoilWords];
     // the leading workgroups of the launch evaluate the macro elements (one thread each,
@@ -590,18 +589,22 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
             if (i < nn && !is_dirichlet<STRUCT>(T, i)) hs[k] = ld_state(a, ve, off + i);
         }
     }
-    const bool has_mel = node_host && model != 1 && tid < kMaxNodeInc;
-    double mel[15];
+    // the host's incident-element data are spread over the threads (one coefficient, or
+    // one vertex head and its perturbation, per thread: few registers each); both
+    // tables are contiguous per cell, so the loads coalesce
+    const bool has_mel = node_host && model != 1;
+    constexpr int kMelSf = kMaxNodeInc * 9, kMelV = kMaxNodeInc * 3;
+    double msf[2] = {0.0, 0.0}, mhv = 0.0, mpv = 0.0;
     if (has_mel) {
-        const double* sf = B.cm_sf + ((size_t)cell * kMaxNodeInc + tid) * 9;
-        const int* vi = B.cm_vi + ((size_t)cell * kMaxNodeInc + tid) * 3;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) mel[k] = sf[k];
-#pragma unroll
-        for (int v = 0; v < 3; ++v) {
-            const int gi = vi[v];
-            mel[9 + v] = state_or(a, ve, gi, S.dirichlet_head);
-            mel[12 + v] = (MODE == 2 && a.v && gi >= 0) ? ve * a.v[gi] : 0.0;
+        for (int r = 0; r < 2; ++r) {
+            const int q = tid + r * bs;
+            if (q < kMelSf) msf[r] = B.cm_sf[(size_t)cell * kMelSf + q];
+        }
+        if (tid < kMelV) {
+            const int gi = B.cm_vi[(size_t)cell * kMelV + tid];
+            mhv = state_or(a, ve, gi, S.dirichlet_head);
+            mpv = (MODE == 2 && a.v && gi >= 0) ? ve * a.v[gi] : 0.0;
         }
     }
     double hb0 = 0.0, hb1 = 0.0;   // tid 0 of a node-hosted cell: host head and its perturbation
@@ -619,7 +622,14 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
     if (tid < nsw) s_sw[tid] = swv;
     if (has_mel) {
 #pragma unroll
-        for (int k = 0; k < 15; ++k) s_mel[tid][k] = mel[k];
+        for (int r = 0; r < 2; ++r) {
+            const int q = tid + r * bs;
+            if (q < kMelSf) s_mel[q / 9][q % 9] = msf[r];
+        }
+        if (tid < kMelV) {
+            s_mel[tid / 3][9 + tid % 3] = mhv;
+            s_mel[tid / 3][12 + tid % 3] = mpv;
+        }
     }
     if (tid == 0) {
         if (!node_host) {
@@ -641,21 +651,19 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
     }
     // MODE 2: interior heads, storage capacities (registers) and conductivities (LDS) of
     // this thread's first two nodes from the recorded expansion; MODE 0/1: their heads
-    double g0r[kLinNPT], str[kLinNPT];   // MODE 2: g0 and storage of this thread's nodes
+    double g0r[kLinNPT], str[kLinNPT];   // MODE 2: G(u) and storage of this thread's nodes
     auto lin_node = [&](int k, const LinNode& x) {
         const int i = tid + k * bs;
-        g0r[k] = 0.0;
         str[k] = 1.0;
         if (i >= nn) return;
-        g0r[k] = x.g0;
         if (is_dirichlet<STRUCT>(T, i)) return;
         // the perturbation itself is the offset from the expansion point (the recorded
         // head is u), so component 0 is not read here
         const double dv = a.v ? ve * x.v : 0.0;
         h[i] = x.u + dv;
-        str[k] = x.c[0] + x.c[1] * dv;
-        KA[i] = x.c[2] + x.c[3] * dv;
-        KB[i] = x.c[4] + x.c[5] * dv;
+        str[k] = x.c[0] + (double)x.d[0] * dv;
+        KA[i] = x.c[1] + (double)x.d[1] * dv;
+        KB[i] = x.c[2] + (double)x.d[2] * dv;
     };
     if constexpr (MODE == 2) {
         lin_node(0, xn[0]);
@@ -717,8 +725,8 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
         if constexpr (MODE == 2) {
             if (b != tid) lin_load_dir(a, NN, off + i, !node_host, xd);
             if (!node_host) d = hi - xd.hb;
-            KA[i] = xd.c[0] + xd.c[1] * d;
-            KB[i] = xd.c[2] + xd.c[3] * d;
+            KA[i] = xd.c[0] + (double)xd.d[0] * d;
+            KB[i] = xd.c[1] + (double)xd.d[1] * d;
         }
     }
     __syncthreads();
@@ -771,6 +779,25 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
         __syncthreads();
     }
 
+    // Exchange Q = sum of the interior net inflows in a fixed order, independent of bs:
+    // 256 slots t take nodes t, t + 256, ... in increasing order; the 64 slots of group g
+    // are added by the wavefront shuffle tree and the four group sums in a fixed tree.
+    // bs is a multiple of 64, so the slots tid + j bs a thread holds for j < 256 / bs
+    // put group g = w + j bs / 64 in one wavefront w = tid / 64, lane = slot - 64 g:
+    // that wavefront sums it (slot_value(j): the thread's j-th slot).
+    auto exchange_groups = [&](auto slot_value) {
+        const int wv = tid >> 6;
+#pragma unroll
+        for (int j2 = 0; j2 < 4; ++j2) {
+            if (j2 * bs >= 256) break;
+            const int g = wv + j2 * (bs >> 6);
+            if (g < 4) {   // uniform across the wavefront: every lane takes part
+                const double r = wave_sum(slot_value(j2));
+                if ((tid & 63) == 0) s_q[g] = r;
+            }
+        }
+    };
+
     // node-centric fluxes and gather: each interior node visits its incident (element,
     // local vertex) pairs in ascending element order and evaluates the two sub-face
     // fluxes it needs, q_j = -K_j (grad h . n_j + n_j,y) (q_j leaves v_j, enters
@@ -781,6 +808,13 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
         else return generic_net(T, i, h, KA, KB, sw);
     };
     if constexpr (MODE == 2) {
+        // G(u) at this thread's interior nodes, loaded now so the loads are in flight
+        // during the gather (G(u) is 0 at micro Dirichlet nodes: nothing to load there)
+#pragma unroll
+        for (int k = 0; k < kLinNPT; ++k) {
+            const int i = tid + k * bs;
+            g0r[k] = (i < nn && !is_dirichlet<STRUCT>(T, i)) ? a.g0[off + i] : 0.0;
+        }
         // the exchange slots (node i -> slot i % 256) this thread owns: with bs dividing
         // 256, node tid + k bs falls in slot tid + (k % (256 / bs)) bs, and each slot's
         // nodes are added in increasing order, as in the LDS version below
@@ -800,16 +834,9 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
                 if (j2 == k % nslot) qs[j2] += net;
             store_fd(B, a, s, off + i, gi, g0r[k]);
         }
-#pragma unroll
-        for (int j2 = 0; j2 < kLinNPT; ++j2)
-            if (j2 < nslot) s_part[tid + j2 * bs] = qs[j2];
+        exchange_groups([&](int j2) { return qs[j2]; });
         __syncthreads();
-        if (tid < 64) {
-            double r4[4];
-#pragma unroll
-            for (int w = 0; w < 4; ++w) r4[w] = wave_sum(s_part[w * 64 + tid]);
-            if (tid == 0) B.Q[cell] = (r4[0] + r4[1]) + (r4[2] + r4[3]);
-        }
+        if (tid == 0) B.Q[cell] = (s_q[0] + s_q[1]) + (s_q[2] + s_q[3]);
         return;
     } else {
         for (int i = tid; i < nn; i += bs) {
@@ -823,21 +850,13 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
         }
     }
     __syncthreads();
-    // exchange Q = sum of interior net inflows in a fixed order, independent of bs:
-    // 256 slots t take nodes t, t+256, ... in order; the first wavefront then sums slots
-    // w*64 + lane with the shuffle tree for w = 0..3 and adds the four in a fixed tree.
-    for (int t2 = tid; t2 < 256; t2 += bs) {
+    exchange_groups([&](int j2) {
         double Qacc = 0.0;
-        for (int i = t2; i < nn; i += 256) Qacc += stl[i];
-        s_part[t2] = Qacc;
-    }
+        for (int i = tid + j2 * bs; i < nn; i += 256) Qacc += stl[i];
+        return Qacc;
+    });
     __syncthreads();
-    if (tid < 64) {
-        double r4[4];
-#pragma unroll
-        for (int w = 0; w < 4; ++w) r4[w] = wave_sum(s_part[w * 64 + tid]);
-        if (tid == 0) B.Q[cell] = (r4[0] + r4[1]) + (r4[2] + r4[3]);
-    }
+    if (tid == 0) B.Q[cell] = (s_q[0] + s_q[1]) + (s_q[2] + s_q[3]);
 }
 
 // ---------------------------------------------------------------- macro nodes

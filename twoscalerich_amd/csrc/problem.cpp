@@ -4,6 +4,7 @@
 // user_data_struct.
 #include "tsr_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -30,6 +31,7 @@ struct tsr_problem {
     exprem_options_struct opt{};
     long long distinct_bytes = 0;
     bool borrowed = false;                   // tmpl holds the caller's micro mesh (not owned)
+    unsigned long long content = 0;          // borrowed problems: checksum of the caller's arrays
 
     ~tsr_problem()
     {
@@ -252,6 +254,37 @@ bool same_problem(const user_data_struct& a, const user_data_struct& b)
            same_soil(a.soilA, b.soilA) && same_soil(a.soilB, b.soilB) && a.boundary_flux == b.boundary_flux &&
            a.model_type == b.model_type;
 }
+// FNV-1a over the caller's arrays that define an adopted problem (mesh geometry and
+// connectivity, BC labels, Keff table, soils, flux, model), so a lookup notices arrays
+// edited in place, or freed and reallocated at the same addresses with other contents.
+unsigned long long content_sum(const user_data_struct& ud)
+{
+    unsigned long long h = 1469598103934665603ull;
+    auto add = [&](const void* p, size_t n) {
+        const unsigned char* b = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < n && b; ++i) { h ^= b[i]; h *= 1099511628211ull; }
+    };
+    const macro_mesh_struct& M = ud.macro_mesh;
+    const micro_mesh_struct& m = ud.micro_mesh;
+    const size_t nv = (size_t)M.no_verts, ne = (size_t)M.no_elements, nn = (size_t)M.no_nodes;
+    const size_t mn = (size_t)m.no_nodes, me = (size_t)m.no_elements;
+    add(M.nodes, sizeof(double) * 2 * nn);
+    add(M.elements, sizeof(int) * nv * ne);
+    add(M.boundary_condition_edges, sizeof(int) * nv * ne);
+    add(m.nodes, sizeof(double) * 2 * mn);
+    add(m.elements, sizeof(int) * 3 * me);
+    add(m.boundary_nodes, sizeof(int) * mn);
+    const Keff_struct& K = ud.effective_conductivity;
+    if (K.no_hvalues > 0) add(K.Keff_values, sizeof(double) * 4 * (size_t)K.no_hvalues);
+    add(&K.epsilonA, sizeof(double));
+    add(&K.logr, sizeof(double));
+    add(&K.logh_first, sizeof(double));
+    add(&ud.soilA, sizeof(soil_struct));
+    add(&ud.soilB, sizeof(soil_struct));
+    add(&ud.boundary_flux, sizeof(double));
+    add(&ud.model_type, sizeof(int));
+    return h;
+}
 }  // namespace
 
 // The view for a user_data_struct: the one a set-up call registered for (elements, N),
@@ -267,7 +300,15 @@ const tsr_batch_view* registry_find(const user_data_struct& ud, std::string& err
     tsr_problem* prev = nullptr;
     for (tsr_problem* q : g_adopted)
         if (v == &q->view) prev = q;
-    if (v && (!prev || same_problem(prev->scen[0].ud, ud))) return v;
+    if (v && !prev) return v;
+    const unsigned long long sum = content_sum(ud);
+    if (prev && same_problem(prev->scen[0].ud, ud) && prev->content == sum) return v;
+    if (prev) {
+        // the caller's arrays changed under the adopted problem: drop it (its device copy
+        // and solvers too) and adopt the current contents
+        g_adopted.erase(std::find(g_adopted.begin(), g_adopted.end(), prev));
+        delete prev;
+    }
     const macro_mesh_struct& M = ud.macro_mesh;
     const micro_mesh_struct& m = ud.micro_mesh;
     if (!M.elements || !m.elements || M.no_nodes <= 0 || M.no_elements <= 0 || m.no_nodes <= 0) {
@@ -303,8 +344,25 @@ const tsr_batch_view* registry_find(const user_data_struct& ud, std::string& err
     p->cell_offset.assign(sc.n_cells, 0);
     set_exprem_options(p->opt);
     finish(p.get(), 0.0);   // registers (elements, N)
+    p->content = sum;
     g_adopted.push_back(p.release());
     return &g_adopted.back()->view;
+}
+
+// Frees the problem (device copy and solvers) adopted for a caller's user_data; 1 when
+// there is none (problems of the set-up calls are freed by tsr_problem_free).
+int release_adopted(const user_data_struct& ud)
+{
+    std::lock_guard<std::mutex> lk(g_adopt_mu);
+    const tsr_batch_view* v = registry_get(ud);
+    for (auto it = g_adopted.begin(); it != g_adopted.end(); ++it)
+        if (v == &(*it)->view) {
+            tsr_problem* q = *it;
+            g_adopted.erase(it);
+            delete q;
+            return 0;
+        }
+    return 1;
 }
 
 void set_error(const std::string& s) { g_err = s; }
@@ -582,6 +640,8 @@ void check_step(double dt, double MinStep)
 }
 
 double check_flag_Gfunc(int flag, double dt) { return flag != 0 ? 0.5 * dt : dt; }
+
+int tsr_release(user_data_struct user_data) { return tsr::release_adopted(user_data); }
 
 int tsr_statistics_write(const char* path, const tsr_step_record* recs, int n, int first_step, int append)
 {

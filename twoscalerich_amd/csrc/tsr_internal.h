@@ -38,6 +38,8 @@ void registry_drop(const tsr_batch_view* v);
 // registry_get, else the reference's own problem around the caller's structs (err set
 // when N fits neither hosting rule)
 const tsr_batch_view* registry_find(const user_data_struct& ud, std::string& err);
+// free the problem adopted for a caller's user_data (0), or 1 if none
+int release_adopted(const user_data_struct& ud);
 // free the solvers that Gfunc_hip / exprem made for a problem's view (solver.cpp)
 void solver_cache_drop(const tsr_batch_view* v);
 
