@@ -270,9 +270,22 @@ int   tsr_mlp_train(void* mlp, const float* x, const float* t, int n, int epochs
  *                      FP32 copies of the Krylov basis columns (FP64 arithmetic; the
  *                      newest column and every FD product stay FP64), halving the
  *                      basis traffic; the basis is then orthonormal to FP32 rounding
- * Returns 0, 1 on unknown bits (or F32BASIS without DCGS2), 2 on allocation failure. */
+ *   TSR_PATH_GRAPH     with DEFERRED: each Arnoldi window (every launch of its iterations
+ *                      and of its readback) is captured into a HIP graph on first use and
+ *                      replayed after, keyed by basis, window bounds, path and state
+ *                      buffer; the same kernels with the same arguments, so the results
+ *                      are those of the path without it, bit for bit.  Windows of a host
+ *                      RHS (exprem_c with a host Gfunc) and before the micro-cell kernel is
+ *                      tuned run uncaptured; the kernel timer does not see replayed
+ *                      windows.
+ * Returns 0, 1 on unknown bits (or F32BASIS without DCGS2, GRAPH without DEFERRED), 2 on
+ * allocation failure. */
 enum { TSR_PATH_PLAIN = 0, TSR_PATH_DCGS2 = 1, TSR_PATH_DEFERRED = 2, TSR_PATH_JVLIN = 4, TSR_PATH_MONOTONE = 8,
-       TSR_PATH_BATCHED = 15, TSR_PATH_F32BASIS = 16 };
+       TSR_PATH_BATCHED = 15, TSR_PATH_F32BASIS = 16, TSR_PATH_GRAPH = 32 };
+
+/* Windows replayed from captured graphs and windows captured since the solver was made
+ * (TSR_PATH_GRAPH). */
+void tsr_solver_graph_stats(const tsr_solver* s, long long* launches, long long* captures);
 int tsr_solver_set_path(tsr_solver* s, int path);
 /* jv = (G(u + eps v) - G(u)) / eps about the solver's state u, with the full RHS
  * (lin = 0) or the TSR_PATH_JVLIN expansion (lin = 1); v, jv host arrays of N. */

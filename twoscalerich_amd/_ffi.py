@@ -91,6 +91,7 @@ GFUNC_T = C.CFUNCTYPE(C.c_int, _dp, _dp, user_data_struct)
 HOST_ELEMENT, HOST_NODE = 0, 1
 CFG_PLAIN_SMALL, CFG_PLAIN, CFG_RAGGED, CFG_SCENARIOS, CFG_LARGE = 0, 1, 2, 3, 4
 PATH_PLAIN, PATH_DCGS2, PATH_DEFERRED, PATH_JVLIN, PATH_MONOTONE, PATH_BATCHED, PATH_F32BASIS = 0, 1, 2, 4, 8, 15, 16
+PATH_GRAPH = 32
 KT_NAMES = ("k_micro (full RHS)", "k_micro (linearised FD product)", "k_macro_node", "k_pass_dca", "k_pass_dcb",
             "k_pass")
 KT_COUNT = len(KT_NAMES)
@@ -143,6 +144,7 @@ def declare(lib):
         "tsr_solver_ktimer": (C.c_int, [P, C.c_int]),
         "tsr_solver_ktimer_read": (C.c_int, [P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _dp, _dp]),
         "tsr_solver_set_step": (C.c_int, [P, C.c_double]),
+        "tsr_solver_graph_stats": (None, [P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
         "tsr_solver_step": (C.c_int, [P, C.c_double, C.POINTER(tsr_step_record)]),
         "tsr_solver_run": (C.c_int, [P, C.c_double, C.c_int, C.POINTER(tsr_run_stats),
                                      C.POINTER(tsr_step_record), C.c_int]),

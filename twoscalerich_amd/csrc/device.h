@@ -151,6 +151,8 @@ hipError_t  launch_gfunc(const DevProblem* p, const GArgs& a, hipStream_t st, bo
 hipError_t  launch_gfunc_part(const DevProblem* p, const GArgs& a, int part, hipStream_t st, bool tune = true);
 // the linearised mode needs micro templates of at most 1024 nodes
 bool        lin_mode_ok(const DevProblem* p);
+// the micro-cell kernel of FD products (the linearised mode when lin) is tuned already
+bool        gfunc_tuned(const DevProblem* p, bool lin);
 // tuned micro-cell launch shape (bs = 0 before the first G)
 void        micro_shape(const DevProblem* p, int* bs, int* wpe);
 

@@ -1398,6 +1398,8 @@ hipError_t launch_gfunc_part(const DevProblem* p, const GArgs& a, int part, hipS
 
 bool lin_mode_ok(const DevProblem* p) { return p->B.nn_max <= 256 * kLinNPT; }
 
+bool gfunc_tuned(const DevProblem* p, bool lin) { return p->micro_bs[lin ? 2 : 0] != 0; }
+
 void micro_shape(const DevProblem* p, int* bs, int* wpe)
 {
     *bs = p->micro_bs[0];

@@ -27,6 +27,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <tuple>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -203,6 +204,10 @@ struct tsr_solver {
     double t_dense = 0.0;     // host seconds in the small dense phi routines
     double t_sync = 0.0;      // host seconds waiting for the stream
     KTimer kt;                // sampled per-kernel timing (off unless tsr_solver_ktimer)
+    // TSR_PATH_GRAPH: instantiated graphs of Arnoldi windows, keyed by what their kernel
+    // arguments depend on (basis, first and last iteration, path bits, state buffer)
+    std::map<std::tuple<int, int, int, int, const double*>, hipGraphExec_t> graphs;
+    long long graph_launches = 0, graph_captures = 0;
 };
 
 namespace {
@@ -220,6 +225,38 @@ hipError_t timed_sync(tsr_solver* s)
     if (e == hipSuccess && !s->kt.pend.empty()) kt_harvest(s->kt);
     return e;
 }
+
+// Stream capture of one Arnoldi window (TSR_PATH_GRAPH): the kernel timer is off while
+// capturing, and a capture left open by an early error return is ended and discarded.
+struct CaptureGuard {
+    hipStream_t st;
+    bool active = false;
+    KTimer* keep = nullptr;
+    explicit CaptureGuard(hipStream_t s) : st(s) {}
+    hipError_t begin()
+    {
+        keep = g_kt;
+        g_kt = nullptr;
+        const hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        active = e == hipSuccess;
+        if (!active) g_kt = keep;
+        return e;
+    }
+    hipError_t end(hipGraph_t* g)
+    {
+        active = false;
+        g_kt = keep;
+        return hipStreamEndCapture(st, g);
+    }
+    ~CaptureGuard()
+    {
+        if (!active) return;
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(st, &g);
+        if (g) hipGraphDestroy(g);
+        g_kt = keep;
+    }
+};
 
 // the solver's kernel timer is active on this thread while it steps
 struct KtScope {
@@ -405,6 +442,21 @@ int arnoldi(tsr_solver* s, int which)
         // (the bad flags were cleared by the previous readback)
         if (upload_mask(s)) return 2;
         const int j0 = j;
+        // TSR_PATH_GRAPH: the window's launches (every iteration and the readback pack)
+        // are captured once per key and replayed; the stream then holds one graph launch
+        // instead of several kernels per iteration.  The micro-cell kernels must be tuned
+        // already (tuning times candidates with events), the RHS must be the device's, and
+        // the kernel timer is off while capturing.
+        const bool graph = (s->path & TSR_PATH_GRAPH) && !s->host_g && gfunc_tuned(s->P, s->path & TSR_PATH_JVLIN);
+        const std::tuple<int, int, int, int, const double*> gkey{which, j0, jt, s->path, s->u};
+        hipGraphExec_t gexec = nullptr;
+        CaptureGuard capture(s->st);
+        if (graph) {
+            auto it = s->graphs.find(gkey);
+            if (it != s->graphs.end()) gexec = it->second;
+            else HIPCHK(capture.begin());
+        }
+        if (gexec) j = jt;   // replayed below
         for (; j < jt; ++j) {
             if (G_call(s, B[j], K.veps, s->g0, K.ieps, B[j + 1], s->mask)) return 2;
             if (dc && f32) {
@@ -428,10 +480,28 @@ int arnoldi(tsr_solver* s, int which)
         }
         // FP32 copy of the window's last column as it stands (the next window's first
         // iteration corrects it and rewrites the copy; a scenario that stops here uses it)
-        if (dc && f32) HIPCHK(launch_to_f32(B32[jt], B[jt], s->M, s->nblk, s->mask, s->st));
+        if (dc && f32 && !gexec) HIPCHK(launch_to_f32(B32[jt], B[jt], s->M, s->nblk, s->mask, s->st));
         // H columns c0..jt-1 changed in this window (DCGS2 also corrects column j0-1)
         const int c0 = dc ? std::max(j0 - 1, 0) : j0;
-        if (read_window(s, K, c0, jt)) return 2;
+        if (graph) {
+            if (!gexec) {
+                HIPCHK(launch_pack_window(K, c0, jt, s->bad, s->mask, s->S, s->mb_d, s->mb_stride, s->st));
+                hipGraph_t gr = nullptr;
+                HIPCHK(capture.end(&gr));
+                const hipError_t ie = hipGraphInstantiate(&gexec, gr, nullptr, nullptr, 0);
+                hipGraphDestroy(gr);
+                HIPCHK(ie);
+                s->graphs[gkey] = gexec;
+                s->graph_captures++;
+            }
+            HIPCHK(hipGraphLaunch(gexec, s->st));
+            s->graph_launches++;
+            HIPCHK(timed_sync(s));
+            for (int q = 0; q < s->S; ++q)
+                if (s->mask_h[q]) s->bad_h[q] = window_part(s, q, c0, jt, 3)[0] != 0.0;
+        } else if (read_window(s, K, c0, jt)) {
+            return 2;
+        }
         for (int q = 0; q < S; ++q) {
             if (!s->mask_h[q]) continue;
             ScState& c = s->sc[q];
@@ -1104,6 +1174,7 @@ static void release_solver(tsr_solver* s)
     if (s->ev1) hipEventDestroy(s->ev1);
     if (s->mlp) mlp_free(s->mlp);
     for (hipEvent_t e : s->kt.all) hipEventDestroy(e);
+    for (auto& g : s->graphs) hipGraphExecDestroy(g.second);
     free_problem(s->P);
     if (s->st) hipStreamDestroy(s->st);
     delete s;
@@ -1262,6 +1333,12 @@ int tsr_solver_ktimer_read(tsr_solver* s, long long* seen, long long* timed, dou
     return 0;
 }
 
+void tsr_solver_graph_stats(const tsr_solver* s, long long* launches, long long* captures)
+{
+    if (launches) *launches = s->graph_launches;
+    if (captures) *captures = s->graph_captures;
+}
+
 int tsr_solver_set_step(tsr_solver* s, double tau)
 {
     if (!(tau > 0.0) || !std::isfinite(tau)) return 1;
@@ -1369,7 +1446,11 @@ int tsr_solver_jv_noalloc(tsr_solver* s, const double* v_host, double eps, int l
 
 int tsr_solver_set_path(tsr_solver* s, int path)
 {
-    if (path & ~(TSR_PATH_BATCHED | TSR_PATH_F32BASIS)) return 1;
+    if (path & ~(TSR_PATH_BATCHED | TSR_PATH_F32BASIS | TSR_PATH_GRAPH)) return 1;
+    if ((path & TSR_PATH_GRAPH) && !(path & TSR_PATH_DEFERRED)) {
+        set_error("TSR_PATH_GRAPH needs TSR_PATH_DEFERRED (a window is what is captured)");
+        return 1;
+    }
     if ((path & TSR_PATH_F32BASIS) && !(path & TSR_PATH_DCGS2)) {
         set_error("TSR_PATH_F32BASIS needs TSR_PATH_DCGS2");
         return 1;
