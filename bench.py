@@ -125,6 +125,9 @@ def main():
                     help="storage of the Krylov basis columns on the batched path (arithmetic is FP64 either way)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only time the G kernels in their three modes on a perturbed initial state (for profilers)")
+    ap.add_argument("--accuracy", type=float, default=0.0, metavar="T",
+                    help="accuracy study instead of the benchmark: to t = T d, a plain run at rtol = atol = 1e-10 "
+                         "as the reference, plain and learned+batched at 1e-6, 1e-7, 1e-8 against it")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -154,6 +157,10 @@ def main():
         print(json.dumps({"gfunc": gfunc_modes(T, fast, args.kernel_reps),
                           "copy_16N_ms": round(copy_ms.value, 4), "copy_bytes": 16 * p.N}), flush=True)
         fast.close()
+        return
+    if args.accuracy > 0:
+        fast.close()
+        accuracy_study(T, F, CT, p, u0, args.accuracy, path_fast)
         return
     plain = T.Solver(p, opt)     # plain path, classical controller (the defaults)
 
@@ -280,6 +287,38 @@ def main():
     print(json.dumps(line), flush=True)
     fast.close()
     plain.close()
+
+
+def accuracy_study(T, F, CT, p, u0, horizon, path_fast):
+    """Global error of each path at t = horizon against a plain run at rtol = atol = 1e-10,
+    at rtol = atol in {1e-6, 1e-7, 1e-8}; one JSON line per run.  Every run starts from
+    InitStep 1e-9 with MinStep 1e-16 (the tightest tolerance's first steps need that)."""
+    def run(tol, learned):
+        opt = p.options()
+        opt.RelTol = opt.AbsTol = tol
+        opt.InitStep, opt.MinStep = 1e-9, 1e-16
+        s = T.Solver(p, opt)
+        if learned:
+            s.set_path(path_fast)
+            CT.attach(s)
+        integrate(T, s, u0, 0.0, 1e-9, 10, "accuracy warm-up")
+        label = f"accuracy {'learned+batched' if learned else 'plain'} tol {tol:g}"
+        w, tot, _ = integrate(T, s, u0, 0.0, horizon, 1000, label)
+        u, _ = s.get_state()
+        s.close()
+        return u, w, tot
+    u_ref, w_ref, t_ref = run(1e-10, False)
+    print(json.dumps({"accuracy_reference": {"t": horizon, "rtol_atol": 1e-10, "path": "plain", "wall_s": round(w_ref, 2),
+                                             "steps": t_ref["steps"], "rhs_evals": t_ref["fevals"]}}), flush=True)
+    for tol in (1e-6, 1e-7, 1e-8):
+        for learned in (False, True):
+            u, w, tot = run(tol, learned)
+            print(json.dumps({"accuracy": {"t": horizon, "rtol_atol": tol,
+                                           "path": "learned+batched" if learned else "plain", "wall_s": round(w, 3),
+                                           "steps": tot["steps"], "rhs_evals": tot["fevals"],
+                                           "rel_err_max": float(np.max(np.abs(u - u_ref)) / np.max(np.abs(u_ref))),
+                                           "rel_err_2norm": float(np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref))}}),
+                  flush=True)
 
 
 def gfunc_modes(T, s, reps):
