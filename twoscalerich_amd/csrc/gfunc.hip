@@ -154,6 +154,9 @@ __device__ __forceinline__ void macro_elem(const DBatch& B, const GArgs& a, int 
     const double ve = a.v ? a.veps[s] : 0.0;
     double hv[4], al[4];
     macro_interp(S, a, ve, e, hv, al);
+    // one sub-face and one half-edge at a time (few registers: this code shares its
+    // kernel's register budget with the micro cells)
+#pragma unroll 1
     for (int j = 0; j < nv; ++j) {
         const double x = S.midx[j * ne + e], y = S.midy[j * ne + e];
         double gx, gy, hip;
@@ -164,8 +167,10 @@ __device__ __forceinline__ void macro_elem(const DBatch& B, const GArgs& a, int 
         const double nx = S.nrm[(2 * j) * ne + e], ny = S.nrm[(2 * j + 1) * ne + e];
         S.qe[e * nv + j] = -((K[0] * gx + K[1] * (gy + 1.0)) * nx + (K[2] * gx + K[3] * (gy + 1.0)) * ny);
     }
+#pragma unroll 1
     for (int p = 0; p < nv; ++p) {
         const int type = S.bce[p * ne + e];
+#pragma unroll 1
         for (int hk = 0; hk < 2; ++hk) {
             double f = 0.0;
             if (type == TSR_BC_FLUX) {
@@ -343,12 +348,13 @@ __device__ __forceinline__ void for_incident(const DTmpl& T, int i, F&& f)
     }
 }
 
-template <bool STRUCT>
+template <bool STRUCT, int R = 0>
 __device__ __forceinline__ bool is_dirichlet(const DTmpl& T, int i)
 {
     if constexpr (STRUCT) {
-        const int iy = div_r(T, i), ix = i - iy * T.r;
-        return ix == 0 || iy == 0 || ix == T.r - 1 || iy == T.r - 1;
+        const int r = R > 0 ? R : T.r;
+        const int iy = R > 0 ? (int)((unsigned)i / (unsigned)R) : div_r(T, i), ix = i - iy * r;
+        return ix == 0 || iy == 0 || ix == r - 1 || iy == r - 1;
     } else {
         return T.bnd[i] == 1;
     }
@@ -406,11 +412,14 @@ __device__ __forceinline__ void pair_struct(double& net, int e, int v0, int v1, 
     __builtin_amdgcn_sched_barrier(0);   // one pair at a time (register pressure)
 }
 
+// R > 0: the grid size is a compile-time constant (neighbour offsets become immediates);
+// R = 0: the template's own r.  The arithmetic is the same either way.
+template <int R = 0>
 __device__ __forceinline__ double struct_net(const DTmpl& T, int i, const double* h, const double* KA,
                                              const double* KB, const unsigned* sw)
 {
-    const int r = T.r, rm = r - 1;
-    const int iy = div_r(T, i), ix = i - iy * r;
+    const int r = R > 0 ? R : T.r, rm = r - 1;
+    const int iy = R > 0 ? (int)((unsigned)i / (unsigned)R) : div_r(T, i), ix = i - iy * r;
     const double d = T.dx;
     const int iLD = i - r - 1, iD = i - r, iL = i - 1, iR = i + 1, iU = i + r, iRU = i + r + 1;
     const int eLD = 2 * ((iy - 1) * rm + ix - 1), eRD = 2 * ((iy - 1) * rm + ix);
@@ -859,6 +868,258 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
     if (tid == 0) B.Q[cell] = (s_q[0] + s_q[1]) + (s_q[2] + s_q[3]);
 }
 
+// ---------------------------------------------------------------- linearised FD products on uniform grids
+// k_micro_lin<R, W, NPT>: the linearised FD product (k_micro mode 2) for problems whose
+// micro templates are all uniform r x r grids, one workgroup of W wavefronts per cell,
+// thread tid owning nodes tid + k 64W (k < NPT).  R > 0 is the grid size as a
+// compile-time constant (every template of the problem has r = R): neighbour offsets
+// and the node -> (ix, iy) split are then immediates; R = 0 reads r per cell.
+//
+// Every load a cell needs of HBM is issued before the first wait: per owned node the
+// recorded conductivities and their derivatives, and at interior nodes the state, the
+// direction, the recorded storage and G(u); the soil words; the macro interpolation data
+// of the host (node-hosted cells) or the host element's interpolant (element-hosted).
+// One memory round trip (two for the macro heads: vertex index, then head) therefore
+// precedes the gather, and with W = 1 the workgroup barriers are a single wavefront's.
+// Per node the arithmetic is that of k_micro mode 2 (same expressions, same order),
+// and the exchange slots are summed in the same order, so the results are the same bit
+// for bit; the Dirichlet nodes are taken by their owner thread instead of by perimeter
+// position, which changes who computes them, not what.
+template <int R, int W, int NPT>
+__global__ __launch_bounds__(64 * W) void k_micro_lin(DBatch B, GArgs a)
+{
+    constexpr int BS = 64 * W;
+    extern __shared__ __align__(16) double smem[];
+    __shared__ double s_bc[6];
+    __shared__ double s_mel[kMaxNodeInc][16];
+    __shared__ double s_q[4];
+    __shared__ unsigned s_sw[kMaxSoilWords];
+    const int mwg = macro_wgs(B, BS);
+    if ((int)blockIdx.x < mwg * B.n_scen) {
+        const int sm = blockIdx.x / mwg;
+        if (!a.active || a.active[sm]) macro_elem(B, a, sm, (blockIdx.x % mwg) * BS + threadIdx.x);
+        return;
+    }
+    const int cell = blockIdx.x - mwg * B.n_scen;
+    const int tid = threadIdx.x;
+    // ---- loads.  First everything indexed by the cell alone (one memory level): the
+    // host's macro interpolation data (every cell has its slots: -1 vertex indices and
+    // zero coefficients where unused), the soil words, the cell's scenario, template
+    // and state offset.  Nothing here waits on the scenario's description.
+    constexpr int kMelSf = kMaxNodeInc * 9, kMelV = kMaxNodeInc * 3;
+    int gi = -1;
+    double msf[2] = {0.0, 0.0};
+    if (tid < kMelV) gi = B.cm_vi[(size_t)cell * kMelV + tid];
+#pragma unroll
+    for (int q2 = 0; q2 < 2; ++q2) {
+        const int q = tid + q2 * BS;
+        if (q < kMelSf) msf[q2] = B.cm_sf[(size_t)cell * kMelSf + q];
+    }
+    const int nsw = B.soil_words;
+    const unsigned swv = tid < nsw ? B.soil_bits[(size_t)cell * nsw + tid] : 0u;
+    const int s = B.cell_scen[cell];
+    const long long off = B.cell_off[cell];
+    const DTmpl& T = B.tmpl[B.cell_tmpl[cell]];
+    if (a.active && !a.active[s]) return;
+    const DScen& S = B.scen[s];
+    const int r = R > 0 ? R : T.r;
+    const int nn = R > 0 ? R * R : T.nn;
+    const int ld = R > 0 ? R * R : B.nn_max;   // LDS stride of the three node arrays
+    double* h = smem;
+    double* KA = h + ld;
+    double* KB = KA + ld;
+    const size_t NN = (size_t)B.N;
+    // per-cell stream bases (uniform): the loads below index them by node only
+    const double* pu = a.u + off;
+    const double* pv = a.v ? a.v + off : nullptr;
+    const double* pg = a.g0 + off;
+    const double* L = a.lin + off;
+    const float* Ld = reinterpret_cast<const float*>(a.lin + kLinComp64 * NN) + off;
+
+    // then the nodes (two levels: offset, then data): every owned node's recorded
+    // conductivities and derivatives; interior nodes' state, direction, recorded storage
+    // and G(u); Dirichlet nodes' interpolating macro element
+    double xu[NPT], xv[NPT], xs[NPT], xa[NPT], xb[NPT], xg[NPT];
+    float ds[NPT], da[NPT], db[NPT];
+    int best[NPT];
+    bool dir[NPT];
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+        const int i = tid + k * BS;
+        xu[k] = xv[k] = xs[k] = xa[k] = xb[k] = xg[k] = 0.0;
+        ds[k] = da[k] = db[k] = 0.0f;
+        best[k] = 0;
+        dir[k] = false;
+        if (i >= nn) continue;
+        dir[k] = is_dirichlet<true, R>(T, i);
+        xa[k] = L[2 * NN + i];
+        xb[k] = L[3 * NN + i];
+        da[k] = Ld[NN + i];
+        db[k] = Ld[2 * NN + i];
+        if (!dir[k]) {
+            xu[k] = pu[i];
+            xv[k] = pv ? pv[i] : 0.0;
+            xs[k] = L[NN + i];
+            ds[k] = Ld[i];
+            xg[k] = pg[i];
+        } else {
+            best[k] = B.dir_elem[(size_t)cell * B.nn_max + i];
+        }
+    }
+    // last, what waits on the scenario's description or on a loaded index: the macro
+    // vertex heads of a node-hosted cell, the host's head, the Dirichlet nodes' template
+    // positions (models 2/3) and recorded heads (element-hosted cells)
+    const int model = S.model;
+    const bool node_host = (S.host == TSR_HOST_NODE);
+    const bool has_mel = node_host && model != 1;
+    const int host = cell - S.first_cell;
+    double mu = 0.0, mv = 0.0;
+    if (has_mel && gi >= 0) {
+        mu = a.u[gi];
+        if (a.v) mv = a.v[gi];
+    }
+    double hu0 = 0.0, hv0 = 0.0;
+    bool hdir = false;
+    if (node_host && tid == 0) {
+        const long long gh = S.u_off + host;
+        hu0 = a.u[gh];
+        if (a.v) hv0 = a.v[gh];
+        hdir = S.dir[host] != 0;
+    }
+    double xh[NPT], vx[NPT], vy[NPT];
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+        const int i = tid + k * BS;
+        xh[k] = vx[k] = vy[k] = 0.0;
+        if (!dir[k]) continue;
+        if (!node_host) xh[k] = L[i];
+        if (model != 1) {
+            vx[k] = T.vec[i];
+            vy[k] = T.vec[nn + i];
+        }
+    }
+    const double ve = a.v ? a.veps[s] : 0.0;
+    // the same expressions as state_or / ld_state: u + ve v, or the Dirichlet head
+    const double mhv = gi < 0 ? S.dirichlet_head : (a.v ? mu + ve * mv : mu);
+    const double mpv = (a.v && gi >= 0) ? ve * mv : 0.0;
+    double hb0 = 0.0, hb1 = 0.0;
+    if (node_host && tid == 0) {
+        const double hu = a.v ? hu0 + ve * hv0 : hu0;
+        const double pu0 = a.v ? ve * hv0 : 0.0;
+        hb0 = hdir ? S.dirichlet_head : hu;
+        hb1 = hdir ? 0.0 : pu0;
+    }
+
+    // ---- LDS: soil words, macro data, interior nodes' heads and conductivities
+    if (tid < nsw) s_sw[tid] = swv;
+    if (has_mel) {
+#pragma unroll
+        for (int q2 = 0; q2 < 2; ++q2) {
+            const int q = tid + q2 * BS;
+            if (q < kMelSf) s_mel[q / 9][q % 9] = msf[q2];
+        }
+        if (tid < kMelV) {
+            s_mel[tid / 3][9 + tid % 3] = mhv;
+            s_mel[tid / 3][12 + tid % 3] = mpv;
+        }
+    }
+    if (tid == 0) {
+        if (!node_host) {
+            double hv[4], al[4];
+            macro_interp(S, a, ve, host, hv, al);
+            if (S.nv == 3) { s_bc[0] = al[0]; s_bc[1] = al[1]; s_bc[2] = 0.0; s_bc[3] = al[2]; }
+            else { s_bc[0] = al[0]; s_bc[1] = al[1]; s_bc[2] = al[2]; s_bc[3] = al[3]; }
+            s_bc[4] = S.cen[host];
+            s_bc[5] = S.cen[S.ne + host];
+        } else {
+            s_bc[2] = hb1;
+            s_bc[3] = hb0;
+            s_bc[4] = S.nodes[host];
+            s_bc[5] = S.nodes[S.nn + host];
+        }
+    }
+    double str[NPT];
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+        const int i = tid + k * BS;
+        str[k] = 1.0;
+        if (i >= nn || dir[k]) continue;
+        const double dv = a.v ? ve * xv[k] : 0.0;
+        h[i] = xu[k] + dv;
+        str[k] = xs[k] + (double)ds[k] * dv;
+        KA[i] = xa[k] + (double)da[k] * dv;
+        KB[i] = xb[k] + (double)db[k] * dv;
+    }
+    __syncthreads();
+
+    // ---- Dirichlet nodes: heads from the macro field, conductivities expanded about the
+    // recorded head by the head's perturbation d
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+        const int i = tid + k * BS;
+        if (i >= nn || !dir[k]) continue;
+        double hi, d = 0.0;
+        if (node_host) {
+            if (model == 1) {
+                hi = s_bc[3];
+                d = s_bc[2];
+            } else {
+                // dir_point's expressions with the prefetched template position
+                const double px = fmin(fmax(s_bc[4] + vx[k], S.box[0]), S.box[1]);
+                const double py = fmin(fmax(s_bc[5] + vy[k], S.box[2]), S.box[3]);
+                double lam[3];
+                macro_bary(s_mel[best[k]], px, py, lam);
+                double ws = 0.0, hsum = 0.0, psum = 0.0;
+#pragma unroll
+                for (int v = 0; v < 3; ++v) {
+                    const double w = fmax(lam[v], 0.0);
+                    ws += w;
+                    hsum += w * s_mel[best[k]][9 + v];
+                    psum += w * s_mel[best[k]][12 + v];
+                }
+                hi = hsum / ws;
+                d = psum / ws;
+            }
+        } else {
+            const double X = (model == 1) ? s_bc[4] : s_bc[4] + vx[k];
+            const double Y = (model == 1) ? s_bc[5] : s_bc[5] + vy[k];
+            hi = s_bc[0] * X + s_bc[1] * Y + s_bc[2] * X * Y + s_bc[3];
+            d = hi - xh[k];
+        }
+        h[i] = hi;
+        KA[i] = xa[k] + (double)da[k] * d;
+        KB[i] = xb[k] + (double)db[k] * d;
+    }
+    __syncthreads();
+
+    // ---- gather, the FD quotient, and the exchange slots (node i -> slot i % 256; the
+    // slots of a group of 64 lie in one wavefront since W divides 4)
+    constexpr int NSLOT = 4 / W;
+    double qs[NSLOT];
+#pragma unroll
+    for (int j2 = 0; j2 < NSLOT; ++j2) qs[j2] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+        const int i = tid + k * BS;
+        if (i >= nn) continue;
+        double gi = 0.0, net = 0.0;
+        if (!dir[k]) {
+            net = struct_net<R>(T, i, h, KA, KB, s_sw);
+            gi = net / str[k];
+        }
+        qs[k % NSLOT] += net;
+        store_fd(B, a, s, off + i, gi, xg[k]);
+    }
+    const int wv = tid >> 6;
+#pragma unroll
+    for (int j2 = 0; j2 < NSLOT; ++j2) {
+        const double q = wave_sum(qs[j2]);
+        if ((tid & 63) == 0) s_q[wv + j2 * W] = q;
+    }
+    __syncthreads();
+    if (tid == 0) B.Q[cell] = (s_q[0] + s_q[1]) + (s_q[2] + s_q[3]);
+}
+
 // ---------------------------------------------------------------- macro nodes
 __global__ __launch_bounds__(256) void k_macro_node(DBatch B, GArgs a)
 {
@@ -908,6 +1169,9 @@ struct DevProblem {
     bool all_structured = false;      // every template is a uniform grid: analytic k_micro
     int micro_bs[3] = {0, 0, 0};      // tuned launch shape per k_micro mode (0 = not tuned yet)
     int micro_wpe[3] = {1, 1, 1};
+    // linearised FD products by k_micro_lin: -1 not used (generic templates), 0 with the
+    // grid size read per cell, R > 0 with R compiled in (every template is R x R)
+    int lin_r = -1;
     double tune_ms[64];               // measured candidates (for reports)
     int tune_n = 0;
     long long n_macro = 0;            // macro nodes of all scenarios (the rest of N are micro nodes)
@@ -1242,6 +1506,14 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
     }
     P->all_structured = !std::getenv("TSR_GENERIC_MICRO");
     for (int t = 0; t < v.n_tmpl; ++t) P->all_structured = P->all_structured && to[t].structured;
+    if (P->all_structured && nn_max <= 1024 && !std::getenv("TSR_MICRO_LIN_OFF")) {
+        P->lin_r = 0;
+        bool same = true;
+        for (int t = 1; t < v.n_tmpl; ++t) same = same && to[t].r == to[0].r;
+        if (same && v.n_tmpl > 0)
+            for (int rr : {11, 16, 21, 31})
+                if (to[0].r == rr) P->lin_r = rr;
+    }
     P->lds = (size_t)4 * nn_max * sizeof(double);
     P->lds2 = (size_t)3 * nn_max * sizeof(double);   // linearised mode: no storage array
     if (P->lds > 64 * 1024) {
@@ -1283,21 +1555,47 @@ static MicroFn micro_fn(bool structured, int mode, int wpe)
     return mode == 2 ? micro_fn<false, 2>(wpe) : mode == 1 ? micro_fn<false, 1>(wpe) : micro_fn<false, 0>(wpe);
 }
 
+// k_micro_lin instances: grid size R (0 = read per cell), W wavefronts per cell, NPT
+// nodes per thread (64 W NPT >= R^2; R = 0 serves templates of up to 1024 nodes)
+struct LinVariant { int R, W, NPT; MicroFn f; };
+static const std::vector<LinVariant>& lin_variants()
+{
+    static const std::vector<LinVariant> v = {
+        {11, 1, 2, k_micro_lin<11, 1, 2>}, {11, 2, 1, k_micro_lin<11, 2, 1>},
+        {16, 1, 4, k_micro_lin<16, 1, 4>}, {16, 2, 2, k_micro_lin<16, 2, 2>}, {16, 4, 1, k_micro_lin<16, 4, 1>},
+        {21, 1, 7, k_micro_lin<21, 1, 7>}, {21, 2, 4, k_micro_lin<21, 2, 4>}, {21, 4, 2, k_micro_lin<21, 4, 2>},
+        {31, 2, 8, k_micro_lin<31, 2, 8>}, {31, 4, 4, k_micro_lin<31, 4, 4>},
+        {0, 2, 8, k_micro_lin<0, 2, 8>}, {0, 4, 4, k_micro_lin<0, 4, 4>},
+    };
+    return v;
+}
+
 static std::vector<const void*> micro_kernels()
 {
     std::vector<const void*> v;
     for (int st = 0; st < 2; ++st)
         for (int mode = 0; mode < 3; ++mode)
             for (int w : kWpe) v.push_back((const void*)micro_fn(st == 1, mode, w));
+    for (const LinVariant& x : lin_variants()) v.push_back((const void*)x.f);
     return v;
 }
 
 static int micro_mode(const GArgs& a) { return a.lin ? 2 : a.rec ? 1 : 0; }
 
+// the k_micro_lin instance of this problem with bs = 64 W threads per cell (nullptr: none)
+static MicroFn lin_fn(const DevProblem* p, int bs)
+{
+    for (const LinVariant& x : lin_variants())
+        if (x.R == p->lin_r && 64 * x.W == bs && 64 * x.W * x.NPT >= p->B.nn_max) return x.f;
+    return nullptr;
+}
+
 static void launch_micro(const DevProblem* p, const GArgs& a, int bs, int wpe, hipStream_t st)
 {
     const DBatch& B = p->B;
-    MicroFn f = micro_fn(p->all_structured, micro_mode(a), wpe);
+    MicroFn f = nullptr;
+    if (micro_mode(a) == 2 && p->lin_r >= 0) f = lin_fn(p, bs);
+    if (!f) f = micro_fn(p->all_structured, micro_mode(a), wpe);
     const int grid = B.n_cells + macro_wgs(B, bs) * B.n_scen;
     hipLaunchKernelGGL(f, dim3(grid), dim3(bs), micro_mode(a) == 2 ? p->lds2 : p->lds, st, B, a);
 }
@@ -1309,9 +1607,17 @@ static void launch_micro(const DevProblem* p, const GArgs& a, int bs, int wpe, h
 // Workgroup sizes the micro-cell kernel admits for a mode: at most 8 strided nodes per
 // thread; the linearised mode also needs bs to divide 256 and at most kLinNPT nodes per
 // thread.
-static std::vector<int> micro_bs_candidates(const DBatch& B, int mode)
+static std::vector<int> micro_bs_candidates(const DevProblem* p, int mode)
 {
+    const DBatch& B = p->B;
     std::vector<int> cand_bs;
+    if (mode == 2 && p->lin_r >= 0) {
+        // TSR_LIN_BS pins the shape (tests of each shape); otherwise every compiled one
+        const char* pin = std::getenv("TSR_LIN_BS");
+        for (int bs : {64, 128, 256})
+            if (lin_fn(p, bs) && (!pin || std::atoi(pin) == bs)) cand_bs.push_back(bs);
+        if (!cand_bs.empty()) return cand_bs;
+    }
     for (int bs : {64, 128, 192, 256, 384, 512, 768, 1024}) {
         const bool ok2 = mode != 2 || (256 % bs == 0 && bs * kLinNPT >= B.nn_max);
         if (bs <= kMicroMaxBS && bs * 8 >= B.nn_max && ok2) cand_bs.push_back(bs);
@@ -1324,14 +1630,16 @@ static hipError_t tune_micro(DevProblem* p, const GArgs& a, hipStream_t st)
 {
     const DBatch& B = p->B;
     const int mode = micro_mode(a);
-    const std::vector<int> cand_bs = micro_bs_candidates(B, mode);
+    const std::vector<int> cand_bs = micro_bs_candidates(p, mode);
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return hipErrorUnknown;
     // register budgets whose code spills to scratch are not candidates: besides being
     // slow, a spilling build of this kernel (mode 1, 8 waves per SIMD) has given
     // wrong results
+    // (k_micro_lin takes no register budget: one candidate)
     std::vector<int> cand_w;
-    for (int w : kWpe) {
+    if (mode == 2 && p->lin_r >= 0 && lin_fn(p, cand_bs[0])) cand_w.push_back(1);
+    else for (int w : kWpe) {
         hipFuncAttributes fa;
         if (hipFuncGetAttributes(&fa, (const void*)micro_fn(p->all_structured, mode, w)) == hipSuccess &&
             fa.localSizeBytes == 0)
@@ -1377,7 +1685,7 @@ hipError_t launch_gfunc_part(const DevProblem* p, const GArgs& a, int part, hipS
             // not tuned yet and tuning not wanted (it creates events): the smallest
             // admissible shape with the compiler's register budget (results do not
             // depend on the shape)
-            launch_micro(p, a, micro_bs_candidates(p->B, mode)[0], 1, st);
+            launch_micro(p, a, micro_bs_candidates(p, mode)[0], 1, st);
             return hipGetLastError();
         }
         // algorithmic bytes: u (and v, G(u) for an FD product) read, the result written,
