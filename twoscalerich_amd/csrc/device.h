@@ -85,6 +85,10 @@ struct DBatch {
     const int* cell_tmpl;
     const int* cell_scen;
     const long long* cell_off;
+    // cell_stride > 0: every cell c starts at cell_off0 + c * cell_stride (one scenario,
+    // one template), so kernels can place a cell without loading its table entries
+    long long cell_off0;
+    int cell_stride;
     const double* discs;
     double* Q;                        // [n_cells]
     int* bad;                         // [n_scen] set when G produced a non-finite value

@@ -917,9 +917,10 @@ __global__ __launch_bounds__(64 * W) void k_micro_lin(DBatch B, GArgs a)
     }
     const int nsw = B.soil_words;
     const unsigned swv = tid < nsw ? B.soil_bits[(size_t)cell * nsw + tid] : 0u;
-    const int s = B.cell_scen[cell];
-    const long long off = B.cell_off[cell];
-    const DTmpl& T = B.tmpl[B.cell_tmpl[cell]];
+    const bool uni = B.cell_stride > 0;   // one scenario, one template, cells in order
+    const int s = uni ? 0 : B.cell_scen[cell];
+    const long long off = uni ? B.cell_off0 + (long long)cell * B.cell_stride : B.cell_off[cell];
+    const DTmpl& T = B.tmpl[uni ? 0 : B.cell_tmpl[cell]];
     if (a.active && !a.active[s]) return;
     const DScen& S = B.scen[s];
     const int r = R > 0 ? R : T.r;
@@ -1482,6 +1483,14 @@ DevProblem* upload_problem(const tsr_batch_view& v, std::string& err)
     B.nn_max = nn_max; B.ne_max = ne_max; B.macro_nn_max = mnn; B.macro_ne_max = mne;
     B.scen = at<DScen>(base, o_S); B.tmpl = at<DTmpl>(base, o_T);
     B.cell_tmpl = at<int>(base, o_ct); B.cell_scen = at<int>(base, o_cs); B.cell_off = at<long long>(base, o_co);
+    B.cell_off0 = v.n_cells > 0 ? v.cell_offset[0] : 0;
+    B.cell_stride = 0;
+    if (v.n_scen == 1 && v.n_tmpl == 1 && v.n_cells > 0) {
+        const long long st = v.tmpl[0].no_nodes;
+        bool lin = true;
+        for (int c = 0; c < v.n_cells && lin; ++c) lin = v.cell_offset[c] == B.cell_off0 + c * st && v.cell_tmpl[c] == 0;
+        if (lin) B.cell_stride = (int)st;
+    }
     B.discs = v.max_discs > 0 ? at<double>(base, o_disc) : nullptr;
     B.Q = at<double>(base, o_Q); B.bad = at<int>(base, o_bad);
     B.dir_elem = at<unsigned char>(base, o_de);
