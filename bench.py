@@ -28,8 +28,10 @@ Extra fields on the JSON line:
                 and the rate they give
   roofline      the dominant kernel of the timed steps (largest summed device time):
                 algorithmic bytes per launch / its average launch duration, against
-                8 TB/s; traffic = FETCH+WRITE bytes per launch from rocprofv3 counter runs
-                (profiles/), scaled by the copy-kernel calibration, or null
+                8 TB/s; traffic = this run's algorithmic bytes per launch x the HBM bytes per
+                algorithmic byte that rocprofv3 FETCH_SIZE / WRITE_SIZE runs measured for
+                the kernel class (profiles/hbm_traffic_plain_workload.json, copy-kernel
+                calibrated), or null
   gfunc         G alone: device time per launch of the micro-cell kernel in its three
                 modes (HIP events) and the HBM rates of its algorithmic bytes (16 B per
                 unknown for G, 32 B for an FD product), roofline_gfunc for the FD product
@@ -57,7 +59,18 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_micro_lin_plain_traffic.json")
+# name -> (problem configuration, description); every one integrates from h0 = -5 m
+WORKLOADS = {
+    "plain": (1, "plain GPU scale: macro 101x101, micro 21x21 per macro node, loam matrix + centred sand disc "
+                 "r=0.25, Dirichlet -0.1 m top, h0=-5 m, t in [0, 1] d"),
+    "ragged": (2, "ragged batch: macro 101x101, micro r in {11,16,21,31} per cell (seeded), 1-3 sand discs per "
+                  "cell, Dirichlet -0.1 m top, h0=-5 m"),
+    "scenarios": (3, "scenario batch: 64 scenarios of macro 41x41 / micro 16x16 with sampled van Genuchten "
+                     "parameters, one launch per kernel for all, h0=-5 m"),
+    "large": (4, "large: macro 201x201, micro 31x31 per macro node, one random sand disc per cell, Krylov "
+                 "dimension up to 40 on the plain path, h0=-5 m"),
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_plain_workload.json")
 T_START = time.perf_counter()
 
 
@@ -125,6 +138,10 @@ def main():
                     help="storage of the Krylov basis columns on the batched path (arithmetic is FP64 either way)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only time the G kernels in their three modes on a perturbed initial state (for profilers)")
+    ap.add_argument("--workload", choices=tuple(WORKLOADS), default="plain",
+                    help="the timed workload (the headline is 'plain'; the others are measured with a bounded --tend)")
+    ap.add_argument("--acc-tols", default="1e-6,1e-7,1e-8",
+                    help="accuracy study: the rtol = atol values of the runs compared with the reference")
     ap.add_argument("--accuracy", type=float, default=0.0, metavar="T",
                     help="accuracy study instead of the benchmark: to t = T d, a plain run at rtol = atol = 1e-10 "
                          "as the reference, plain and learned+batched at 1e-6, 1e-7, 1e-8 against it")
@@ -140,7 +157,7 @@ def main():
     from twoscalerich_amd import controller as CT
     L = T.lib()
 
-    p = T.Problem.config(F.CFG_PLAIN)
+    p = T.Problem.config(WORKLOADS[args.workload][0])
     opt = p.options()
     tend = args.tend if args.tend > 0 else p.tend()
     u0 = p.initial()
@@ -160,7 +177,7 @@ def main():
         return
     if args.accuracy > 0:
         fast.close()
-        accuracy_study(T, F, CT, p, u0, args.accuracy, path_fast)
+        accuracy_study(T, F, CT, p, u0, args.accuracy, path_fast, [float(x) for x in args.acc_tols.split(",")])
         return
     plain = T.Solver(p, opt)     # plain path, classical controller (the defaults)
 
@@ -223,11 +240,17 @@ def main():
                          "share_among_timed_classes": round(avg_ms * v["seen"] / total_ms, 4)}
     dom = max(kernels, key=lambda n: kernels[n]["share_among_timed_classes"])
     dk = kernels[dom]
-    traffic = None
+    # HBM traffic per launch: the counter runs' (FETCH_SIZE + WRITE_SIZE, copy-calibrated)
+    # bytes per algorithmic byte of this kernel class, times this run's algorithmic bytes
+    # per launch (the DCGS2 passes' bytes grow with the Krylov index, so the ratio, not a
+    # byte count, carries over from the counter run's shorter horizon)
+    traffic, traffic_ratio = None, None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             tj = json.load(f)
-        traffic = tj.get("per_class", {}).get(dom, {}).get("bytes_per_launch")
+        traffic_ratio = tj.get("per_class", {}).get(dom, {}).get("traffic_over_algorithmic")
+        if traffic_ratio is not None:
+            traffic = round(traffic_ratio * dk["bytes_per_launch"])
 
     # G kernels per mode (HIP events on the solver stream), on the final state
     gf = gfunc_modes(T, fast, args.kernel_reps)
@@ -257,8 +280,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "plain GPU scale: macro 101x101, micro 21x21 per macro node, loam matrix + "
-                               "centred sand disc r=0.25, Dirichlet -0.1 m top, h0=-5 m, t in [0, 1] d",
+        "config": {"workload": WORKLOADS[args.workload][1],
                    "N": p.N, "t_end_days": tend, "rtol": opt.RelTol, "atol": opt.AbsTol,
                    "max_krylov_plain": opt.MaxKrylov, "controller": "learned (Krylov range [5, 60])",
                    "path": "batched (DCGS2" + (" on FP32 copies of the basis" if args.basis == "f32" else
@@ -275,6 +297,7 @@ def main():
                          "class during the timed steps; bytes are algorithmic (reads and writes the kernel must do)",
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": dk["GB_per_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dk["GB_per_s"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_over_algorithmic": traffic_ratio,
                      "algorithmic_bytes_per_launch": dk["bytes_per_launch"], "avg_launch_ms": dk["avg_launch_ms"],
                      "copy_kernel_GB_per_s": round(copy_gbs, 1)},
         "roofline_gfunc": {"kernel": "k_micro (linearised FD product)", "bound": "hbm",
@@ -289,10 +312,10 @@ def main():
     plain.close()
 
 
-def accuracy_study(T, F, CT, p, u0, horizon, path_fast):
+def accuracy_study(T, F, CT, p, u0, horizon, path_fast, tols):
     """Global error of each path at t = horizon against a plain run at rtol = atol = 1e-10,
-    at rtol = atol in {1e-6, 1e-7, 1e-8}; one JSON line per run.  Every run starts from
-    InitStep 1e-9 with MinStep 1e-16 (the tightest tolerance's first steps need that)."""
+    at rtol = atol in tols; one JSON line per run.  Every run starts from InitStep 1e-9
+    with MinStep 1e-16 (the tightest tolerance's first steps need that)."""
     def run(tol, learned):
         opt = p.options()
         opt.RelTol = opt.AbsTol = tol
@@ -310,7 +333,7 @@ def accuracy_study(T, F, CT, p, u0, horizon, path_fast):
     u_ref, w_ref, t_ref = run(1e-10, False)
     print(json.dumps({"accuracy_reference": {"t": horizon, "rtol_atol": 1e-10, "path": "plain", "wall_s": round(w_ref, 2),
                                              "steps": t_ref["steps"], "rhs_evals": t_ref["fevals"]}}), flush=True)
-    for tol in (1e-6, 1e-7, 1e-8):
+    for tol in tols:
         for learned in (False, True):
             u, w, tot = run(tol, learned)
             print(json.dumps({"accuracy": {"t": horizon, "rtol_atol": tol,

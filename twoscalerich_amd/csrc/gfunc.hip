@@ -1645,13 +1645,17 @@ static hipError_t tune_micro(DevProblem* p, const GArgs& a, hipStream_t st)
     // register budgets whose code spills to scratch are not candidates: besides being
     // slow, a spilling build of this kernel (mode 1, 8 waves per SIMD) has given
     // wrong results
-    // (k_micro_lin takes no register budget: one candidate)
+    // (k_micro_lin takes no register budget: one candidate).  TSR_MICRO_WPE pins the
+    // budget of k_micro (tests of each budget; a pinned budget is taken even when its
+    // code spills).
     std::vector<int> cand_w;
+    const char* pin_w = std::getenv("TSR_MICRO_WPE");
     if (mode == 2 && p->lin_r >= 0 && lin_fn(p, cand_bs[0])) cand_w.push_back(1);
     else for (int w : kWpe) {
         hipFuncAttributes fa;
-        if (hipFuncGetAttributes(&fa, (const void*)micro_fn(p->all_structured, mode, w)) == hipSuccess &&
-            fa.localSizeBytes == 0)
+        if (pin_w ? std::atoi(pin_w) == w
+                  : hipFuncGetAttributes(&fa, (const void*)micro_fn(p->all_structured, mode, w)) == hipSuccess &&
+                        fa.localSizeBytes == 0)
             cand_w.push_back(w);
     }
     if (cand_w.empty()) cand_w.push_back(1);
