@@ -61,6 +61,8 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 # name -> (problem configuration, description); every one integrates from h0 = -5 m
 WORKLOADS = {
+    "small": (0, "plain CPU-runnable: macro 21x21, micro 11x11 per macro node, centred sand disc r=0.25, "
+                 "Dirichlet -0.1 m top, h0=-5 m, t in [0, 0.1] d"),
     "plain": (1, "plain GPU scale: macro 101x101, micro 21x21 per macro node, loam matrix + centred sand disc "
                  "r=0.25, Dirichlet -0.1 m top, h0=-5 m, t in [0, 1] d"),
     "ragged": (2, "ragged batch: macro 101x101, micro r in {11,16,21,31} per cell (seeded), 1-3 sand discs per "
@@ -134,6 +136,8 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--ktimer-every", type=int, default=4, help="time every E-th launch of each kernel class")
     ap.add_argument("--chunk", type=int, default=250, help="EXPREM steps between progress checks")
+    ap.add_argument("--graph", action="store_true",
+                    help="batched path with its Arnoldi windows captured into HIP graphs (TSR_PATH_GRAPH)")
     ap.add_argument("--basis", choices=("f64", "f32"), default="f64",
                     help="storage of the Krylov basis columns on the batched path (arithmetic is FP64 either way)")
     ap.add_argument("--kernels-only", action="store_true",
@@ -161,7 +165,7 @@ def main():
     opt = p.options()
     tend = args.tend if args.tend > 0 else p.tend()
     u0 = p.initial()
-    path_fast = F.PATH_BATCHED | (F.PATH_F32BASIS if args.basis == "f32" else 0)
+    path_fast = F.PATH_BATCHED | (F.PATH_F32BASIS if args.basis == "f32" else 0) | (F.PATH_GRAPH if args.graph else 0)
 
     fast = T.Solver(p, opt)
     fast.set_path(path_fast)
@@ -284,7 +288,8 @@ def main():
                    "N": p.N, "t_end_days": tend, "rtol": opt.RelTol, "atol": opt.AbsTol,
                    "max_krylov_plain": opt.MaxKrylov, "controller": "learned (Krylov range [5, 60])",
                    "path": "batched (DCGS2" + (" on FP32 copies of the basis" if args.basis == "f32" else
-                                               ", FP64 basis") + ", deferred Krylov checks, linearised FD products)",
+                                               ", FP64 basis") + ", deferred Krylov checks, linearised FD products" +
+                           (", Arnoldi windows replayed from HIP graphs" if args.graph else "") + ")",
                    "steps_requested": args.steps, "budget_s": args.budget_s},
         "walls_s": [round(x, 3) for x in walls],
         "exprem": {"accepted_steps": st["steps"], "rejected": st["rejects"], "rhs_evals": st["fevals"],

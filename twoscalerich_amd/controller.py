@@ -71,9 +71,11 @@ def records_to_arrays(recs):
 
 
 # cost model of one step on the batched path, in units of one linearised RHS evaluation
-# (plain workload on MI355X: 0.195 ms per linearised G, 7.2 us per basis-column read)
-COST_C0 = 4.7      # G(u) with the expansion record, G(U2), host syncs, controller
-COST_R = 0.037     # one basis-column read in a Gram-Schmidt pass / one RHS evaluation
+# (plain workload on MI355X: 94 us per linearised G with its macro nodes, 7 us per
+# basis-column read in the DCGS2 passes, 0.78 ms per step for G(u) with the expansion
+# record, G(U2), the step combinations, the controller and the host round trips)
+COST_C0 = 8.3      # per-step work outside the Arnoldi iterations
+COST_R = 0.075     # one basis-column read in a Gram-Schmidt pass / one RHS evaluation
 
 
 def step_cost(m, mD, c0=COST_C0, r=COST_R):
@@ -276,10 +278,11 @@ def collect(solver, tend, max_steps, chunk=500, progress=None):
 
 
 # ------------------------------------------------------------------ training pipeline
-# (workload, seed, max solver steps): the plain and ragged workloads from t = 0, and
-# three batches of 64 scenarios with sampled soils (192 soil draws).
-PLAN = ((F.CFG_PLAIN, 0, 30000), (F.CFG_RAGGED, 0, 6000),
-        (F.CFG_SCENARIOS, 11, 1200), (F.CFG_SCENARIOS, 12, 1200), (F.CFG_SCENARIOS, 13, 1200))
+# (workload, seed, max solver steps): the plain workload from t = 0 to t_end, the ragged
+# workload from t = 0, and three batches of 64 scenarios with sampled soils (192 soil
+# draws).
+PLAN = ((F.CFG_PLAIN, 0, 40000), (F.CFG_RAGGED, 0, 10000),
+        (F.CFG_SCENARIOS, 11, 2500), (F.CFG_SCENARIOS, 12, 2500), (F.CFG_SCENARIOS, 13, 2500))
 
 
 def generate(plan=PLAN, path=F.PATH_BATCHED, log=print):
@@ -394,7 +397,7 @@ def main(argv=None):
     if a.generate:
         t0 = time.time()
         d = generate()
-        np.savez(a.data, **d)
+        np.savez_compressed(a.data, **d)
         print(f"[controller] {len(d['X'])} records in {time.time() - t0:.1f} s -> {a.data}", flush=True)
     if a.train:
         with np.load(a.data, allow_pickle=False) as z:

@@ -175,6 +175,9 @@ enum KtClass {
     KT_COUNT = 6
 };
 bool kt_begin(int cls, double bytes, hipStream_t st);
+// share of the unknowns the current scenario mask keeps active (1 without a mask): the
+// launchers scale a masked launch's bytes by it
+double kt_active_fraction();
 void kt_end(hipStream_t st);
 
 // Row blocks of the vector kernels: block b covers rows [r0, r1) of scenario scen.

@@ -1704,13 +1704,15 @@ hipError_t launch_gfunc_part(const DevProblem* p, const GArgs& a, int part, hipS
         // algorithmic bytes: u (and v, G(u) for an FD product) read, the result written,
         // per micro node
         const double bpu = 8.0 * (2 + (a.v ? 1 : 0) + (a.g0 ? 1 : 0));
-        const bool kt = kt_begin(mode == 2 ? KT_MICRO_LIN : KT_MICRO_FULL, bpu * (double)(B.N - p->n_macro), st);
+        const double fr = a.active ? kt_active_fraction() : 1.0;
+        const bool kt = kt_begin(mode == 2 ? KT_MICRO_LIN : KT_MICRO_FULL, fr * bpu * (double)(B.N - p->n_macro), st);
         launch_micro(p, a, p->micro_bs[mode], p->micro_wpe[mode], st);
         if (kt) kt_end(st);
     } else {
         dim3 gn((B.macro_nn_max + 63) / 64, B.n_scen);
         const double bpu = 8.0 * (2 + (a.v ? 1 : 0) + (a.g0 ? 1 : 0));
-        const bool kt = kt_begin(KT_MACRO_NODE, bpu * (double)p->n_macro, st);
+        const double fr = a.active ? kt_active_fraction() : 1.0;
+        const bool kt = kt_begin(KT_MACRO_NODE, fr * bpu * (double)p->n_macro, st);
         hipLaunchKernelGGL(k_macro_node, gn, dim3(64), 0, st, B, a);
         if (kt) kt_end(st);
     }

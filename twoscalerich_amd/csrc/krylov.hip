@@ -737,7 +737,7 @@ hipError_t launch_pass(int mode, const double* const* cols, int nk, double* w, c
 {
     // algorithmic bytes: nk columns and w read; w written back by the two axpy passes;
     // u read by the last
-    const double rows = (double)nblk * kVecBS * kVecR;
+    const double rows = (double)nblk * kVecBS * kVecR * (active ? kt_active_fraction() : 1.0);
     const double bytes = 8.0 * rows * (nk + 1 + (mode != PASS_DOTS ? 1 : 0) + (mode == PASS_AXPY_NORM ? 1 : 0));
     const bool kt = kt_begin(KT_PASS, bytes, st);
     hipLaunchKernelGGL(k_pass, dim3(nblk), dim3(kVecBS), 0, st, mode, cols, nk, w, u, coef, ldc, M, part, ldp, active,
@@ -811,7 +811,7 @@ static hipError_t dcgs2(int nk, const T* const* cols, const double* Rj_in, T* Rj
     const int j = nk - 1;
     // algorithmic bytes per row: k_pass_dca reads columns k < j (as T), R_j and w;
     // k_pass_dcb reads them and u, writes w and (j > 0 or all_j) the corrected R_j (as T)
-    const double rows = (double)nblk * kVecBS * kVecR;
+    const double rows = (double)nblk * kVecBS * kVecR * (active ? kt_active_fraction() : 1.0);
     const double bca = rows * (sizeof(T) * (double)j + 16.0);
     const double bcb = rows * (sizeof(T) * (double)j + 32.0 + ((all_j || j > 0) ? sizeof(T) : 0));
     bool kt = kt_begin(KT_DCA, bca, st);

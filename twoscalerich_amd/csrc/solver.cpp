@@ -72,6 +72,10 @@ struct KTimer {
 
 namespace tsr {
 thread_local KTimer* g_kt = nullptr;   // the timer of the solver stepping on this thread
+// share of the unknowns in scenarios the current activity mask keeps (masked launches
+// move only those scenarios' bytes)
+thread_local double g_kt_frac = 1.0;
+double kt_active_fraction() { return g_kt_frac; }
 
 static hipEvent_t kt_event(KTimer& k)
 {
@@ -335,6 +339,11 @@ int G_at(tsr_solver* s, const double* x, double* out, const int* active)
 int upload_mask(tsr_solver* s)
 {
     const int fail_code = 2;
+    {
+        double on = 0.0, all = 0.0;
+        for (int q = 0; q < s->S; ++q) { all += (double)s->rows_h[q]; if (s->mask_h[q]) on += (double)s->rows_h[q]; }
+        g_kt_frac = all > 0.0 ? on / all : 1.0;
+    }
     if (s->mask_dev_h == s->mask_h) return 0;   // the device copy is current
     HIPCHK(hipMemcpyAsync(s->mask, s->mask_h.data(), sizeof(int) * s->S, hipMemcpyHostToDevice, s->st));
     s->mask_dev_h = s->mask_h;
