@@ -115,3 +115,28 @@ def test_learned_controller_gain():
     # tight-tolerance solution at this time, profiles/)
     assert d_l <= 1e-5 and d_c <= 1e-5
     p.close()
+
+
+def test_accuracy_gate_against_tight_tolerance():
+    """The metric's accuracy condition on the plain GPU-scale workload: at rtol = atol =
+    1e-7 the plain path and the learned + batched path both land within 1e-6 (relative,
+    max norm) of a plain-path solution at rtol = atol = 1e-10, at t = 0.005 d (every run
+    from InitStep 1e-9, as the tight run needs)."""
+    p = T.Problem.config(F.CFG_PLAIN)
+    tend = 0.005
+
+    def opts(tol):
+        o = p.options()
+        o.RelTol = o.AbsTol = tol
+        o.InitStep, o.MinStep = 1e-9, 1e-16
+        return o
+
+    uref, sref = _run(p, F.PATH_PLAIN, tend, opt=opts(1e-10), max_steps=10 ** 6)
+    up, sp = _run(p, F.PATH_PLAIN, tend, opt=opts(1e-7))
+    ul, sl = _run(p, F.PATH_BATCHED, tend, controller="learned", opt=opts(1e-7))
+    scale = np.max(np.abs(uref))
+    ep, el = np.max(np.abs(up - uref)) / scale, np.max(np.abs(ul - uref)) / scale
+    print(f"[batched] accuracy at t={tend}: reference {sref.steps} steps; rtol 1e-7: plain {sp.steps} steps "
+          f"max-rel {ep:.2e}, learned+batched {sl.steps} steps max-rel {el:.2e}", flush=True)
+    assert ep <= 1e-6 and el <= 1e-6
+    p.close()

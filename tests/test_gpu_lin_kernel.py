@@ -85,3 +85,43 @@ def test_lin_kernel_every_workgroup_shape():
           flush=True)
     for o in outs[1:]:
         assert np.array_equal(o, outs[0])
+
+
+BUDGET_CASES = [("plain r21", F.CFG_PLAIN, 7, 21, 0), ("ragged", F.CFG_RAGGED, 9, 0, 0),
+                ("large discs r31", F.CFG_LARGE, 5, 31, 0)]
+
+
+@pytest.mark.parametrize("name,cfg,macro_n,micro_r,n_scen", BUDGET_CASES, ids=[c[0] for c in BUDGET_CASES])
+def test_lin_kernel_every_register_budget(name, cfg, macro_n, micro_r, n_scen):
+    """k_micro_lin under both register budgets the tuner times (the compiler's, and 8
+    waves per SIMD, whose larger shapes spill a few registers to scratch), for each
+    multi-wavefront shape: the products must not change, bit for bit, and must equal
+    k_micro's.  TSR_LIN_BS and TSR_MICRO_WPE, read at a problem's first tuning, pin the
+    shape and the budget."""
+    print(f"[lin-budgets] {name}", flush=True)
+    ref = None
+    rng = np.random.default_rng(17)
+    for bs, wpe, lin_kernel in ((0, 1, False), (128, 1, True), (128, 8, True), (256, 1, True), (256, 8, True)):
+        env = {"TSR_MICRO_WPE": str(wpe)}
+        if bs:
+            env["TSR_LIN_BS"] = str(bs)
+        os.environ.update(env)
+        try:
+            p = _problem(cfg, macro_n, micro_r, n_scen, lin_kernel)
+            if ref is None:
+                u = p.initial() + 0.8 * rng.standard_normal(p.N)
+                v = rng.uniform(-1.0, 1.0, p.N)
+            s = T.Solver(p)
+            s.set_state(u, 0.0)
+            out = s.jv(v, 1e-7, lin=True)
+            s.close(); p.close()
+        finally:
+            for k in env:
+                os.environ.pop(k, None)
+        if ref is None:
+            ref = out
+            assert np.all(np.isfinite(ref))
+            continue
+        nd = int(np.sum(out != ref))
+        print(f"[lin-budgets] {name}: bs {bs}, waves/EU {wpe}: {nd} entries differ from k_micro", flush=True)
+        assert nd == 0

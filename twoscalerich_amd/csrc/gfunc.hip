@@ -869,24 +869,40 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
 }
 
 // ---------------------------------------------------------------- linearised FD products on uniform grids
-// k_micro_lin<R, W, NPT>: the linearised FD product (k_micro mode 2) for problems whose
-// micro templates are all uniform r x r grids, one workgroup of W wavefronts per cell,
-// thread tid owning nodes tid + k 64W (k < NPT).  R > 0 is the grid size as a
-// compile-time constant (every template of the problem has r = R): neighbour offsets
-// and the node -> (ix, iy) split are then immediates; R = 0 reads r per cell.
+// Perimeter position p (0 <= p < 4 (r - 1)) of an r x r grid -> node index: the bottom
+// row, the top row, then the left and the right column without their corners.
+__device__ __forceinline__ int perim_node(int p, int r)
+{
+    if (p < r) return p;
+    if (p < 2 * r) return (r - 1) * r + p - r;
+    const int q = p - 2 * r;
+    return q < r - 2 ? (q + 1) * r : (q - r + 3) * r + r - 1;
+}
+
+// Largest grid size of a k_micro_lin problem (templates of at most 1024 nodes).
+constexpr int kLinMaxR = 32;
+
+// k_micro_lin<R, W, NPT, WPE>: the linearised FD product (k_micro mode 2) for problems
+// whose micro templates are all uniform r x r grids, one workgroup of W wavefronts per
+// cell, thread tid owning nodes tid + k 64W (k < NPT) and perimeter positions
+// tid + q 64W.  R > 0 is the grid size as a compile-time constant (every template of
+// the problem has r = R): neighbour offsets and the node -> (ix, iy) split are then
+// immediates; R = 0 reads r per cell.  WPE is the register budget (waves per SIMD).
 //
 // Every load a cell needs of HBM is issued before the first wait: per owned node the
 // recorded conductivities and their derivatives, and at interior nodes the state, the
-// direction, the recorded storage and G(u); the soil words; the macro interpolation data
-// of the host (node-hosted cells) or the host element's interpolant (element-hosted).
-// One memory round trip (two for the macro heads: vertex index, then head) therefore
-// precedes the gather, and with W = 1 the workgroup barriers are a single wavefront's.
-// Per node the arithmetic is that of k_micro mode 2 (same expressions, same order),
-// and the exchange slots are summed in the same order, so the results are the same bit
-// for bit; the Dirichlet nodes are taken by their owner thread instead of by perimeter
-// position, which changes who computes them, not what.
-template <int R, int W, int NPT>
-__global__ __launch_bounds__(64 * W) void k_micro_lin(DBatch B, GArgs a)
+// direction, the recorded storage and G(u); per perimeter position the Dirichlet node's
+// interpolation data; the soil words; the macro interpolation data of the host
+// (node-hosted cells) or the host element's interpolant (element-hosted).  One memory
+// round trip (two for the macro heads: vertex index, then head) therefore precedes the
+// gather, and with W = 1 the workgroup barriers are a single wavefront's.  The Dirichlet
+// nodes are evaluated densely by perimeter position (4 (r - 1) of them: 2 of 8
+// wavefront passes at r = 21, 64 W = 256), their owners parking the recorded
+// conductivities in the nodes' LDS slots.  Per node the arithmetic is that of k_micro
+// mode 2 (same expressions, same order), and the exchange slots are summed in the same
+// order, so the results are the same bit for bit.
+template <int R, int W, int NPT, int WPE>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE))) void k_micro_lin(DBatch B, GArgs a)
 {
     constexpr int BS = 64 * W;
     extern __shared__ __align__(16) double smem[];
@@ -938,18 +954,21 @@ __global__ __launch_bounds__(64 * W) void k_micro_lin(DBatch B, GArgs a)
     const float* Ld = reinterpret_cast<const float*>(a.lin + kLinComp64 * NN) + off;
 
     // then the nodes (two levels: offset, then data): every owned node's recorded
-    // conductivities and derivatives; interior nodes' state, direction, recorded storage
-    // and G(u); Dirichlet nodes' interpolating macro element
-    double xu[NPT], xv[NPT], xs[NPT], xa[NPT], xb[NPT], xg[NPT];
-    float ds[NPT], da[NPT], db[NPT];
-    int best[NPT];
+    // conductivities and derivatives, and interior nodes' state, direction, recorded
+    // storage and G(u) (n0..n3, nf: zero at Dirichlet nodes).  The Dirichlet nodes are
+    // also taken densely by perimeter slots, thread tid having perimeter positions
+    // tid + q BS: their template position, interpolating macro element and (for
+    // element-hosted cells) recorded head.
+    double n0[NPT], n1[NPT], n2[NPT], n3[NPT], xa[NPT], xb[NPT];
+    float nf[NPT], da[NPT], db[NPT];
     bool dir[NPT];
+    const int model = S.model;
+    const bool node_host = (S.host == TSR_HOST_NODE);
 #pragma unroll
     for (int k = 0; k < NPT; ++k) {
         const int i = tid + k * BS;
-        xu[k] = xv[k] = xs[k] = xa[k] = xb[k] = xg[k] = 0.0;
-        ds[k] = da[k] = db[k] = 0.0f;
-        best[k] = 0;
+        n0[k] = n1[k] = n2[k] = n3[k] = xa[k] = xb[k] = 0.0;
+        nf[k] = da[k] = db[k] = 0.0f;
         dir[k] = false;
         if (i >= nn) continue;
         dir[k] = is_dirichlet<true, R>(T, i);
@@ -958,20 +977,31 @@ __global__ __launch_bounds__(64 * W) void k_micro_lin(DBatch B, GArgs a)
         da[k] = Ld[NN + i];
         db[k] = Ld[2 * NN + i];
         if (!dir[k]) {
-            xu[k] = pu[i];
-            xv[k] = pv ? pv[i] : 0.0;
-            xs[k] = L[NN + i];
-            ds[k] = Ld[i];
-            xg[k] = pg[i];
-        } else {
-            best[k] = B.dir_elem[(size_t)cell * B.nn_max + i];
+            n0[k] = pu[i];
+            n1[k] = pv ? pv[i] : 0.0;
+            n2[k] = L[NN + i];
+            nf[k] = Ld[i];
+            n3[k] = pg[i];
         }
     }
-    // last, what waits on the scenario's description or on a loaded index: the macro
-    // vertex heads of a node-hosted cell, the host's head, the Dirichlet nodes' template
-    // positions (models 2/3) and recorded heads (element-hosted cells)
-    const int model = S.model;
-    const bool node_host = (S.host == TSR_HOST_NODE);
+    constexpr int NPER = (4 * ((R > 0 ? R : kLinMaxR) - 1) + BS - 1) / BS;
+    const int nper = 4 * (r - 1);
+    double bx[NPER], by[NPER], bh[NPER];
+    int be[NPER];
+#pragma unroll
+    for (int q = 0; q < NPER; ++q) {
+        const int p = tid + q * BS;
+        bx[q] = by[q] = bh[q] = 0.0;
+        be[q] = 0;
+        if (p >= nper) continue;
+        const int i = perim_node(p, r);
+        be[q] = B.dir_elem[(size_t)cell * B.nn_max + i];
+        bx[q] = T.vec[i];
+        by[q] = T.vec[nn + i];
+        if (!node_host) bh[q] = L[i];
+    }
+    // last, what waits on a loaded index: the macro vertex heads of a node-hosted cell,
+    // the host's head
     const bool has_mel = node_host && model != 1;
     const int host = cell - S.first_cell;
     double mu = 0.0, mv = 0.0;
@@ -986,18 +1016,6 @@ __global__ __launch_bounds__(64 * W) void k_micro_lin(DBatch B, GArgs a)
         hu0 = a.u[gh];
         if (a.v) hv0 = a.v[gh];
         hdir = S.dir[host] != 0;
-    }
-    double xh[NPT], vx[NPT], vy[NPT];
-#pragma unroll
-    for (int k = 0; k < NPT; ++k) {
-        const int i = tid + k * BS;
-        xh[k] = vx[k] = vy[k] = 0.0;
-        if (!dir[k]) continue;
-        if (!node_host) xh[k] = L[i];
-        if (model != 1) {
-            vx[k] = T.vec[i];
-            vy[k] = T.vec[nn + i];
-        }
     }
     const double ve = a.v ? a.veps[s] : 0.0;
     // the same expressions as state_or / ld_state: u + ve v, or the Dirichlet head
@@ -1039,15 +1057,23 @@ __global__ __launch_bounds__(64 * W) void k_micro_lin(DBatch B, GArgs a)
             s_bc[5] = S.nodes[S.nn + host];
         }
     }
+    // (a Dirichlet node parks its recorded conductivities and derivatives in its own
+    // LDS slots until its head is known, so they hold no registers across the barrier)
     double str[NPT];
 #pragma unroll
     for (int k = 0; k < NPT; ++k) {
         const int i = tid + k * BS;
         str[k] = 1.0;
-        if (i >= nn || dir[k]) continue;
-        const double dv = a.v ? ve * xv[k] : 0.0;
-        h[i] = xu[k] + dv;
-        str[k] = xs[k] + (double)ds[k] * dv;
+        if (i >= nn) continue;
+        if (dir[k]) {
+            h[i] = __hiloint2double(__float_as_int(db[k]), __float_as_int(da[k]));
+            KA[i] = xa[k];
+            KB[i] = xb[k];
+            continue;
+        }
+        const double dv = a.v ? ve * n1[k] : 0.0;
+        h[i] = n0[k] + dv;
+        str[k] = n2[k] + (double)nf[k] * dv;
         KA[i] = xa[k] + (double)da[k] * dv;
         KB[i] = xb[k] + (double)db[k] * dv;
     }
@@ -1056,9 +1082,10 @@ __global__ __launch_bounds__(64 * W) void k_micro_lin(DBatch B, GArgs a)
     // ---- Dirichlet nodes: heads from the macro field, conductivities expanded about the
     // recorded head by the head's perturbation d
 #pragma unroll
-    for (int k = 0; k < NPT; ++k) {
-        const int i = tid + k * BS;
-        if (i >= nn || !dir[k]) continue;
+    for (int q = 0; q < NPER; ++q) {
+        const int p = tid + q * BS;
+        if (p >= nper) continue;
+        const int i = perim_node(p, r);
         double hi, d = 0.0;
         if (node_host) {
             if (model == 1) {
@@ -1066,30 +1093,31 @@ __global__ __launch_bounds__(64 * W) void k_micro_lin(DBatch B, GArgs a)
                 d = s_bc[2];
             } else {
                 // dir_point's expressions with the prefetched template position
-                const double px = fmin(fmax(s_bc[4] + vx[k], S.box[0]), S.box[1]);
-                const double py = fmin(fmax(s_bc[5] + vy[k], S.box[2]), S.box[3]);
+                const double px = fmin(fmax(s_bc[4] + bx[q], S.box[0]), S.box[1]);
+                const double py = fmin(fmax(s_bc[5] + by[q], S.box[2]), S.box[3]);
                 double lam[3];
-                macro_bary(s_mel[best[k]], px, py, lam);
+                macro_bary(s_mel[be[q]], px, py, lam);
                 double ws = 0.0, hsum = 0.0, psum = 0.0;
 #pragma unroll
                 for (int v = 0; v < 3; ++v) {
                     const double w = fmax(lam[v], 0.0);
                     ws += w;
-                    hsum += w * s_mel[best[k]][9 + v];
-                    psum += w * s_mel[best[k]][12 + v];
+                    hsum += w * s_mel[be[q]][9 + v];
+                    psum += w * s_mel[be[q]][12 + v];
                 }
                 hi = hsum / ws;
                 d = psum / ws;
             }
         } else {
-            const double X = (model == 1) ? s_bc[4] : s_bc[4] + vx[k];
-            const double Y = (model == 1) ? s_bc[5] : s_bc[5] + vy[k];
+            const double X = (model == 1) ? s_bc[4] : s_bc[4] + bx[q];
+            const double Y = (model == 1) ? s_bc[5] : s_bc[5] + by[q];
             hi = s_bc[0] * X + s_bc[1] * Y + s_bc[2] * X * Y + s_bc[3];
-            d = hi - xh[k];
+            d = hi - bh[q];
         }
+        const double pd = h[i], ka = KA[i], kb = KB[i];
         h[i] = hi;
-        KA[i] = xa[k] + (double)da[k] * d;
-        KB[i] = xb[k] + (double)db[k] * d;
+        KA[i] = ka + (double)__int_as_float(__double2loint(pd)) * d;
+        KB[i] = kb + (double)__int_as_float(__double2hiint(pd)) * d;
     }
     __syncthreads();
 
@@ -1109,7 +1137,7 @@ __global__ __launch_bounds__(64 * W) void k_micro_lin(DBatch B, GArgs a)
             gi = net / str[k];
         }
         qs[k % NSLOT] += net;
-        store_fd(B, a, s, off + i, gi, xg[k]);
+        store_fd(B, a, s, off + i, gi, n3[k]);
     }
     const int wv = tid >> 6;
 #pragma unroll
@@ -1565,19 +1593,25 @@ static MicroFn micro_fn(bool structured, int mode, int wpe)
 }
 
 // k_micro_lin instances: grid size R (0 = read per cell), W wavefronts per cell, NPT
-// nodes per thread (64 W NPT >= R^2; R = 0 serves templates of up to 1024 nodes)
-struct LinVariant { int R, W, NPT; MicroFn f; };
+// nodes per thread (64 W NPT >= R^2; R = 0 serves templates of up to 1024 nodes), each
+// with the compiler's register budget (f) and, for W >= 2, a budget of 8 waves per SIMD
+// (f8: at most 64 VGPRs)
+struct LinVariant { int R, W, NPT; MicroFn f, f8; };
+#define TSR_LIN1(R, W, NPT) {R, W, NPT, k_micro_lin<R, W, NPT, 1>, nullptr}
+#define TSR_LIN2(R, W, NPT) {R, W, NPT, k_micro_lin<R, W, NPT, 1>, k_micro_lin<R, W, NPT, 8>}
 static const std::vector<LinVariant>& lin_variants()
 {
     static const std::vector<LinVariant> v = {
-        {11, 1, 2, k_micro_lin<11, 1, 2>}, {11, 2, 1, k_micro_lin<11, 2, 1>},
-        {16, 1, 4, k_micro_lin<16, 1, 4>}, {16, 2, 2, k_micro_lin<16, 2, 2>}, {16, 4, 1, k_micro_lin<16, 4, 1>},
-        {21, 1, 7, k_micro_lin<21, 1, 7>}, {21, 2, 4, k_micro_lin<21, 2, 4>}, {21, 4, 2, k_micro_lin<21, 4, 2>},
-        {31, 2, 8, k_micro_lin<31, 2, 8>}, {31, 4, 4, k_micro_lin<31, 4, 4>},
-        {0, 2, 8, k_micro_lin<0, 2, 8>}, {0, 4, 4, k_micro_lin<0, 4, 4>},
+        TSR_LIN1(11, 1, 2), TSR_LIN2(11, 2, 1),
+        TSR_LIN1(16, 1, 4), TSR_LIN2(16, 2, 2), TSR_LIN2(16, 4, 1),
+        TSR_LIN1(21, 1, 7), TSR_LIN2(21, 2, 4), TSR_LIN2(21, 4, 2),
+        TSR_LIN2(31, 2, 8), TSR_LIN2(31, 4, 4),
+        TSR_LIN2(0, 2, 8), TSR_LIN2(0, 4, 4),
     };
     return v;
 }
+#undef TSR_LIN1
+#undef TSR_LIN2
 
 static std::vector<const void*> micro_kernels()
 {
@@ -1585,17 +1619,22 @@ static std::vector<const void*> micro_kernels()
     for (int st = 0; st < 2; ++st)
         for (int mode = 0; mode < 3; ++mode)
             for (int w : kWpe) v.push_back((const void*)micro_fn(st == 1, mode, w));
-    for (const LinVariant& x : lin_variants()) v.push_back((const void*)x.f);
+    for (const LinVariant& x : lin_variants()) {
+        v.push_back((const void*)x.f);
+        if (x.f8) v.push_back((const void*)x.f8);
+    }
     return v;
 }
 
 static int micro_mode(const GArgs& a) { return a.lin ? 2 : a.rec ? 1 : 0; }
 
-// the k_micro_lin instance of this problem with bs = 64 W threads per cell (nullptr: none)
-static MicroFn lin_fn(const DevProblem* p, int bs)
+// the k_micro_lin instance of this problem with bs = 64 W threads per cell and register
+// budget wpe (1 or 8 waves per SIMD; nullptr: none)
+static MicroFn lin_fn(const DevProblem* p, int bs, int wpe)
 {
     for (const LinVariant& x : lin_variants())
-        if (x.R == p->lin_r && 64 * x.W == bs && 64 * x.W * x.NPT >= p->B.nn_max) return x.f;
+        if (x.R == p->lin_r && 64 * x.W == bs && 64 * x.W * x.NPT >= p->B.nn_max)
+            return wpe == 1 ? x.f : wpe == 8 ? x.f8 : nullptr;
     return nullptr;
 }
 
@@ -1603,7 +1642,7 @@ static void launch_micro(const DevProblem* p, const GArgs& a, int bs, int wpe, h
 {
     const DBatch& B = p->B;
     MicroFn f = nullptr;
-    if (micro_mode(a) == 2 && p->lin_r >= 0) f = lin_fn(p, bs);
+    if (micro_mode(a) == 2 && p->lin_r >= 0) f = lin_fn(p, bs, wpe);
     if (!f) f = micro_fn(p->all_structured, micro_mode(a), wpe);
     const int grid = B.n_cells + macro_wgs(B, bs) * B.n_scen;
     hipLaunchKernelGGL(f, dim3(grid), dim3(bs), micro_mode(a) == 2 ? p->lds2 : p->lds, st, B, a);
@@ -1624,7 +1663,7 @@ static std::vector<int> micro_bs_candidates(const DevProblem* p, int mode)
         // TSR_LIN_BS pins the shape (tests of each shape); otherwise every compiled one
         const char* pin = std::getenv("TSR_LIN_BS");
         for (int bs : {64, 128, 256})
-            if (lin_fn(p, bs) && (!pin || std::atoi(pin) == bs)) cand_bs.push_back(bs);
+            if (lin_fn(p, bs, 1) && (!pin || std::atoi(pin) == bs)) cand_bs.push_back(bs);
         if (!cand_bs.empty()) return cand_bs;
     }
     for (int bs : {64, 128, 192, 256, 384, 512, 768, 1024}) {
@@ -1642,16 +1681,20 @@ static hipError_t tune_micro(DevProblem* p, const GArgs& a, hipStream_t st)
     const std::vector<int> cand_bs = micro_bs_candidates(p, mode);
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return hipErrorUnknown;
-    // register budgets whose code spills to scratch are not candidates: besides being
-    // slow, a spilling build of this kernel (mode 1, 8 waves per SIMD) has given
-    // wrong results
-    // (k_micro_lin takes no register budget: one candidate).  TSR_MICRO_WPE pins the
-    // budget of k_micro (tests of each budget; a pinned budget is taken even when its
-    // code spills).
+    // register budgets of k_micro whose code spills to scratch are not candidates:
+    // besides being slow, a spilling build of this kernel (mode 1, 8 waves per SIMD) has
+    // given wrong results.  k_micro_lin's budgets (1 and, for W >= 2, 8 waves per SIMD)
+    // are all timed: its 8-wave builds of the larger shapes spill a few registers and
+    // can still be the fastest, and every budget gives the same products bit for bit
+    // (tested).  TSR_MICRO_WPE pins the budget (tests of each budget; a pinned budget is
+    // taken even when its code spills).
     std::vector<int> cand_w;
     const char* pin_w = std::getenv("TSR_MICRO_WPE");
-    if (mode == 2 && p->lin_r >= 0 && lin_fn(p, cand_bs[0])) cand_w.push_back(1);
-    else for (int w : kWpe) {
+    const bool lin = mode == 2 && p->lin_r >= 0 && lin_fn(p, cand_bs[0], 1);
+    if (lin) {
+        for (int w : {1, 8})
+            if (!pin_w || std::atoi(pin_w) == w) cand_w.push_back(w);
+    } else for (int w : kWpe) {
         hipFuncAttributes fa;
         if (pin_w ? std::atoi(pin_w) == w
                   : hipFuncGetAttributes(&fa, (const void*)micro_fn(p->all_structured, mode, w)) == hipSuccess &&
@@ -1664,6 +1707,7 @@ static hipError_t tune_micro(DevProblem* p, const GArgs& a, hipStream_t st)
     if (mode == 0) p->tune_n = 0;
     for (int bs : cand_bs)
         for (int w : cand_w) {
+            if (lin && !lin_fn(p, bs, w)) continue;   // no 8-wave build of one-wavefront shapes
             launch_micro(p, a, bs, w, st);
             hipEventRecord(e0, st);
             for (int r = 0; r < 3; ++r) launch_micro(p, a, bs, w, st);
@@ -1679,6 +1723,7 @@ static hipError_t tune_micro(DevProblem* p, const GArgs& a, hipStream_t st)
         }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
+    if (best == 1e300) best_w = 1;   // a pinned budget no shape of this problem has
     p->micro_bs[mode] = best_bs;
     p->micro_wpe[mode] = best_w;
     launch_micro(p, a, best_bs, best_w, st);
