@@ -295,7 +295,10 @@ __global__ __launch_bounds__(kVecBS) void k_pass_dcb(const T* const* cols, const
                                                     const int* active, double at, double rt)
 {
     __shared__ double red[kVecBS / 64];
-    const int b = blockIdx.x;
+    // rows in the reverse order of k_pass_dca's: the blocks that start first find the
+    // rows k_pass_dca read last still in the memory-side cache (each block's rows and
+    // partials are its own, so the order changes no result)
+    const int b = gridDim.x - 1 - blockIdx.x;
     const int s = M.scen[b];
     if (active && !active[s]) return;
     const long long r0 = M.r0[b], r1 = M.r1[b];
