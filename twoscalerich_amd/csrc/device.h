@@ -256,8 +256,11 @@ hipError_t launch_fill_eps(double* de, int n_scen, double eps, hipStream_t st);
 // bad flags they read
 hipError_t launch_pack_window(const KrylovState& K, int c0, int jt, int* bad, const int* active, int n_scen,
                               double* out, int stride, hipStream_t st);
-hipError_t launch_pack_small(const double* res, int* bad, const double* eps, const double* stats, int n_scen,
-                             double* out, hipStream_t st);
+// One small readback per scenario, kPackSmall doubles: res[0..7], the bad flag (then
+// cleared), eps, the macro-state summary (stats, 6), the controller's output (ctl, 2)
+constexpr int kPackSmall = 18;
+hipError_t launch_pack_small(const double* res, int* bad, const double* eps, const double* stats, const float* ctl,
+                             int n_scen, double* out, hipStream_t st);
 // out[s*6 + 0..5]: mean, min, max macro head; mean, min, max matrix-soil saturation
 hipError_t launch_macro_stats(const DScen* scen, int n_scen, const double* u, double* out, hipStream_t st);
 

@@ -696,12 +696,12 @@ __global__ void k_pack_window(KrylovState K, int c0, int jt, int* bad, const int
 
 // Per-scenario scalars (scenario s at out + s * 16): res[0..7], bad, eps, stats[0..5]
 // (eps and stats only when given).
-__global__ void k_pack_small(const double* res, int* bad, const double* eps, const double* stats, int n_scen,
-                             double* out)
+__global__ void k_pack_small(const double* res, int* bad, const double* eps, const double* stats, const float* ctl,
+                             int n_scen, double* out)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n_scen) return;
-    double* o = out + (size_t)s * 16;
+    double* o = out + (size_t)s * kPackSmall;
 #pragma unroll
     for (int k = 0; k < 8; ++k) o[k] = res[(size_t)s * 8 + k];
     o[8] = (double)bad[s];
@@ -710,6 +710,10 @@ __global__ void k_pack_small(const double* res, int* bad, const double* eps, con
     if (stats) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) o[10 + k] = stats[(size_t)s * 6 + k];
+    }
+    if (ctl) {
+        o[16] = ctl[2 * s];
+        o[17] = ctl[2 * s + 1];
     }
 }
 
@@ -721,10 +725,10 @@ hipError_t launch_pack_window(const KrylovState& K, int c0, int jt, int* bad, co
     return hipGetLastError();
 }
 
-hipError_t launch_pack_small(const double* res, int* bad, const double* eps, const double* stats, int n_scen,
-                             double* out, hipStream_t st)
+hipError_t launch_pack_small(const double* res, int* bad, const double* eps, const double* stats, const float* ctl,
+                             int n_scen, double* out, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_pack_small, dim3((n_scen + 63) / 64), dim3(64), 0, st, res, bad, eps, stats, n_scen, out);
+    hipLaunchKernelGGL(k_pack_small, dim3((n_scen + 63) / 64), dim3(64), 0, st, res, bad, eps, stats, ctl, n_scen, out);
     return hipGetLastError();
 }
 

@@ -257,6 +257,42 @@ int mlp_set(Mlp* M, const float* params, const float* mean, const float* stdv)
 bool mlp_ready(const Mlp* M) { return M && M->ready; }
 
 // Raw features in, raw outputs out (standardisation of inputs with the stored mean/std).
+// Standardised macro-state features X[s][kStat0 + k] = ((float)stats[s][k] - mean) / stdv,
+// the arithmetic mlp_infer does on the host (FP64 -> FP32 cast, FP32 subtract and
+// correctly rounded divide), so both give the network the same input bit for bit.
+constexpr int kStat0 = 10, kNStat = 6;
+struct StatNorm { float mean[kNStat], stdv[kNStat]; };
+__global__ void k_feat_stats(float* X, const double* stats, int n, StatNorm sn)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+#pragma unroll
+    for (int k = 0; k < kNStat; ++k)
+        X[(size_t)s * kIn + kStat0 + k] = ((float)stats[(size_t)s * kNStat + k] - sn.mean[k]) / sn.stdv[k];
+}
+
+// Inference without a host round trip of its own: features 0..9 from the host (feats,
+// standardised here), 10..15 from the macro-state summary already on the device
+// (stats, n x 6 FP64), output left in mlp_output(M) ([n][2]) for the caller's next
+// readback.
+int mlp_infer_device(Mlp* M, const float* feats, int n, const double* stats, hipStream_t st)
+{
+    if (n <= 0) return 0;
+    if (ensure_cap(M, n)) return 2;
+    std::vector<float> xs((size_t)n * kIn, 0.0f);
+    for (int s = 0; s < n; ++s)
+        for (int k = 0; k < kStat0; ++k) xs[(size_t)s * kIn + k] = (feats[(size_t)s * kIn + k] - M->mean[k]) / M->stdv[k];
+    if (hipMemcpyAsync(M->X, xs.data(), sizeof(float) * xs.size(), hipMemcpyHostToDevice, st) != hipSuccess) return 2;
+    StatNorm sn;
+    for (int k = 0; k < kNStat; ++k) { sn.mean[k] = M->mean[kStat0 + k]; sn.stdv[k] = M->stdv[kStat0 + k]; }
+    hipLaunchKernelGGL(k_feat_stats, dim3((n + 63) / 64), dim3(64), 0, st, M->X, stats, n, sn);
+    hipLaunchKernelGGL(k_mlp_forward, dim3(std::min(64, (n + kMlpWaves - 1) / kMlpWaves)), dim3(64 * kMlpWaves), 0, st,
+                       M->P, M->X, n, M->Y);
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+const float* mlp_output(const Mlp* M) { return M->Y; }
+
 int mlp_infer(Mlp* M, const float* feats, int n, float* out, hipStream_t st)
 {
     if (n <= 0) return 0;

@@ -50,6 +50,8 @@ Mlp*  mlp_create();
 void  mlp_free(Mlp*);
 int   mlp_set(Mlp*, const float* params, const float* mean, const float* stdv);
 int   mlp_infer(Mlp*, const float* feats, int n, float* out, hipStream_t st);
+int   mlp_infer_device(Mlp*, const float* feats, int n, const double* stats, hipStream_t st);
+const float* mlp_output(const Mlp*);
 bool  mlp_ready(const Mlp*);
 }  // namespace tsr
 
@@ -57,17 +59,29 @@ using namespace tsr;
 
 // Sampled per-kernel device timing (tsr_solver_ktimer): an event pair around every
 // `every`-th launch of each timed class, harvested whenever the solver has synchronised
-// its stream (every recorded pair is then complete).
+// its stream (every recorded pair is then complete).  While an Arnoldi window is being
+// captured into a HIP graph the pairs are recorded as external event nodes of the graph
+// (their own events, sampled by a capture-time count): every replay records them again,
+// and the replay hands them, with the window's launch counts, to the pending list.
 struct KTimer {
     int every = 0;
     long long seen[KT_COUNT] = {}, timed[KT_COUNT] = {};
     double ms[KT_COUNT] = {}, bytes[KT_COUNT] = {};
-    struct Pend { hipEvent_t a, b; int cls; double bytes; };
+    struct Pend { hipEvent_t a, b; int cls; double bytes; bool graph; };
     std::vector<Pend> pend;
     std::vector<hipEvent_t> pool, all;
     int open = -1;
     double open_bytes = 0.0;
     hipEvent_t open_a = nullptr;
+    bool capturing = false;
+    std::vector<Pend> cap;                 // pairs of the window being captured
+    long long cap_seen[KT_COUNT] = {};     // its launches per class
+    long long cap_n[KT_COUNT] = {};        // sampling count across captures
+};
+// what a captured window's replay contributes to the timer
+struct GraphTiming {
+    std::vector<KTimer::Pend> pairs;
+    long long seen[KT_COUNT] = {};
 };
 
 namespace tsr {
@@ -86,10 +100,27 @@ static hipEvent_t kt_event(KTimer& k)
     return e;
 }
 
+// an event of its own for a graph (destroyed with the solver, never pooled)
+static hipEvent_t kt_graph_event(KTimer& k)
+{
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    k.all.push_back(e);
+    return e;
+}
+
 bool kt_begin(int cls, double bytes, hipStream_t st)
 {
     KTimer* k = g_kt;
     if (!k || k->every <= 0 || cls < 0 || cls >= KT_COUNT || k->open >= 0) return false;
+    if (k->capturing) {
+        k->cap_seen[cls]++;
+        if (k->cap_n[cls]++ % k->every) return false;
+        hipEvent_t a = kt_graph_event(*k);
+        if (!a || hipEventRecordWithFlags(a, st, hipEventRecordExternal) != hipSuccess) return false;
+        k->open = cls; k->open_a = a; k->open_bytes = bytes;
+        return true;
+    }
     if (k->seen[cls]++ % k->every) return false;
     hipEvent_t a = kt_event(*k);
     if (!a || hipEventRecord(a, st) != hipSuccess) return false;
@@ -101,8 +132,15 @@ void kt_end(hipStream_t st)
 {
     KTimer* k = g_kt;
     if (!k || k->open < 0) return;
+    if (k->capturing) {
+        hipEvent_t b = kt_graph_event(*k);
+        if (b && hipEventRecordWithFlags(b, st, hipEventRecordExternal) == hipSuccess)
+            k->cap.push_back({k->open_a, b, k->open, k->open_bytes, true});
+        k->open = -1;
+        return;
+    }
     hipEvent_t b = kt_event(*k);
-    if (b && hipEventRecord(b, st) == hipSuccess) k->pend.push_back({k->open_a, b, k->open, k->open_bytes});
+    if (b && hipEventRecord(b, st) == hipSuccess) k->pend.push_back({k->open_a, b, k->open, k->open_bytes, false});
     else k->pool.push_back(k->open_a);
     k->open = -1;
 }
@@ -115,7 +153,7 @@ static void kt_harvest(KTimer& k)
         if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
             k.ms[p.cls] += ms; k.bytes[p.cls] += p.bytes; k.timed[p.cls]++;
         }
-        k.pool.push_back(p.a); k.pool.push_back(p.b);
+        if (!p.graph) { k.pool.push_back(p.a); k.pool.push_back(p.b); }
     }
     k.pend.clear();
 }
@@ -203,6 +241,7 @@ struct tsr_solver {
     double* mb_h = nullptr;     // mapped pinned readback buffer (host view) ...
     double* mb_d = nullptr;     // ... and its device address
     int mb_stride = 0;          // doubles per scenario
+    std::vector<float> ctl_h;   // learned controller's output per scenario (2), read with the step start
     int path = TSR_PATH_PLAIN;
     double* linrec = nullptr;   // TSR_PATH_JVLIN: expansion point of the FD products (kLinBytesPerNode x N bytes)
     double t_dense = 0.0;     // host seconds in the small dense phi routines
@@ -210,7 +249,7 @@ struct tsr_solver {
     KTimer kt;                // sampled per-kernel timing (off unless tsr_solver_ktimer)
     // TSR_PATH_GRAPH: instantiated graphs of Arnoldi windows, keyed by what their kernel
     // arguments depend on (basis, first and last iteration, path bits, state buffer)
-    std::map<std::tuple<int, int, int, int, const double*>, hipGraphExec_t> graphs;
+    std::map<std::tuple<int, int, int, int, const double*>, std::pair<hipGraphExec_t, GraphTiming>> graphs;
     long long graph_launches = 0, graph_captures = 0;
 };
 
@@ -230,26 +269,34 @@ hipError_t timed_sync(tsr_solver* s)
     return e;
 }
 
-// Stream capture of one Arnoldi window (TSR_PATH_GRAPH): the kernel timer is off while
-// capturing, and a capture left open by an early error return is ended and discarded.
+// Stream capture of one Arnoldi window (TSR_PATH_GRAPH): the kernel timer, when on,
+// records its sampled pairs into the graph (KTimer::capturing); a capture left open by an
+// early error return is ended and discarded.
 struct CaptureGuard {
     hipStream_t st;
     bool active = false;
-    KTimer* keep = nullptr;
     explicit CaptureGuard(hipStream_t s) : st(s) {}
     hipError_t begin()
     {
-        keep = g_kt;
-        g_kt = nullptr;
+        if (g_kt) {
+            g_kt->capturing = true;
+            g_kt->cap.clear();
+            for (int c = 0; c < KT_COUNT; ++c) g_kt->cap_seen[c] = 0;
+        }
         const hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
         active = e == hipSuccess;
-        if (!active) g_kt = keep;
+        if (!active && g_kt) g_kt->capturing = false;
         return e;
     }
-    hipError_t end(hipGraph_t* g)
+    // ends the capture; the window's timer pairs and launch counts go to *gt
+    hipError_t end(hipGraph_t* g, GraphTiming* gt)
     {
         active = false;
-        g_kt = keep;
+        if (g_kt) {
+            g_kt->capturing = false;
+            gt->pairs.swap(g_kt->cap);
+            for (int c = 0; c < KT_COUNT; ++c) gt->seen[c] = g_kt->cap_seen[c];
+        }
         return hipStreamEndCapture(st, g);
     }
     ~CaptureGuard()
@@ -258,7 +305,7 @@ struct CaptureGuard {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(st, &g);
         if (g) hipGraphDestroy(g);
-        g_kt = keep;
+        if (g_kt) { g_kt->capturing = false; g_kt->cap.clear(); g_kt->open = -1; }
     }
 };
 
@@ -360,17 +407,21 @@ int clear_bad(tsr_solver* s)
 // Read back the per-scenario results (res, 8 slots) and bad flags, plus the FD step
 // and the macro statistics when eps / stats are given: one pack kernel into the mapped
 // buffer, one sync.
-int read_iteration(tsr_solver* s, KrylovState& K, double* eps_out = nullptr, bool stats = false)
+// ctl: the controller's device output ([S][2] floats) to read back with the rest (into ctl_h)
+int read_iteration(tsr_solver* s, KrylovState& K, double* eps_out = nullptr, bool stats = false,
+                   const float* ctl = nullptr)
 {
     const int fail_code = 2;
-    HIPCHK(launch_pack_small(K.res, s->bad, eps_out ? K.eps : nullptr, stats ? s->mstats : nullptr, s->S, s->mb_d, s->st));
+    HIPCHK(launch_pack_small(K.res, s->bad, eps_out ? K.eps : nullptr, stats ? s->mstats : nullptr, ctl, s->S, s->mb_d,
+                             s->st));
     HIPCHK(timed_sync(s));
     for (int q = 0; q < s->S; ++q) {
-        const double* o = s->mb_h + (size_t)q * 16;
+        const double* o = s->mb_h + (size_t)q * kPackSmall;
         std::memcpy(&s->res_h[(size_t)q * 8], o, sizeof(double) * 8);
         s->bad_h[q] = o[8] != 0.0;
         if (eps_out) eps_out[q] = o[9];
         if (stats) std::memcpy(s->sc[q].mstat, o + 10, sizeof(double) * 6);
+        if (ctl) { s->ctl_h[(size_t)q * 2] = (float)o[16]; s->ctl_h[(size_t)q * 2 + 1] = (float)o[17]; }
     }
     return 0;
 }
@@ -459,10 +510,11 @@ int arnoldi(tsr_solver* s, int which)
         const bool graph = (s->path & TSR_PATH_GRAPH) && !s->host_g && gfunc_tuned(s->P, s->path & TSR_PATH_JVLIN);
         const std::tuple<int, int, int, int, const double*> gkey{which, j0, jt, s->path, s->u};
         hipGraphExec_t gexec = nullptr;
+        GraphTiming* gtime = nullptr;
         CaptureGuard capture(s->st);
         if (graph) {
             auto it = s->graphs.find(gkey);
-            if (it != s->graphs.end()) gexec = it->second;
+            if (it != s->graphs.end()) { gexec = it->second.first; gtime = &it->second.second; }
             else HIPCHK(capture.begin());
         }
         if (gexec) j = jt;   // replayed below
@@ -496,15 +548,23 @@ int arnoldi(tsr_solver* s, int which)
             if (!gexec) {
                 HIPCHK(launch_pack_window(K, c0, jt, s->bad, s->mask, s->S, s->mb_d, s->mb_stride, s->st));
                 hipGraph_t gr = nullptr;
-                HIPCHK(capture.end(&gr));
+                GraphTiming gt;
+                HIPCHK(capture.end(&gr, &gt));
                 const hipError_t ie = hipGraphInstantiate(&gexec, gr, nullptr, nullptr, 0);
                 hipGraphDestroy(gr);
                 HIPCHK(ie);
-                s->graphs[gkey] = gexec;
+                auto& slot = s->graphs[gkey];
+                slot.first = gexec;
+                slot.second = std::move(gt);
+                gtime = &slot.second;
                 s->graph_captures++;
             }
             HIPCHK(hipGraphLaunch(gexec, s->st));
             s->graph_launches++;
+            if (g_kt && gtime) {   // this replay's timer pairs (complete at the sync below)
+                for (const auto& p : gtime->pairs) g_kt->pend.push_back(p);
+                for (int c = 0; c < KT_COUNT; ++c) g_kt->seen[c] += gtime->seen[c];
+            }
             HIPCHK(timed_sync(s));
             for (int q = 0; q < s->S; ++q)
                 if (s->mask_h[q]) s->bad_h[q] = window_part(s, q, c0, jt, 3)[0] != 0.0;
@@ -596,6 +656,7 @@ int arnoldi(tsr_solver* s, int which)
 }
 
 int features_and_proposal(tsr_solver* s, const std::vector<int>& act);
+void build_features(const ScState& c, float* f);
 
 }  // namespace
 
@@ -630,8 +691,18 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
     if (launch_reduce(RED_START, s->part, s->ldp, 3, s->blk_first, S, s->scen_rows, s->KV, 0, s->mask, s->st) != hipSuccess)
         return -1;
     HIPCHK(launch_macro_stats(batch_of(s->P).scen, S, s->u, s->mstats, s->st));
+    // learned controller: its inference is enqueued here (the macro-state features come
+    // from the summary just launched, the rest from the host's step history) and its
+    // output comes back with the step-start readback
+    const bool learned = s->ctrl_mode == 1 && s->mlp && mlp_ready(s->mlp);
+    if (learned) {
+        std::vector<float> feat((size_t)S * 16, 0.0f);
+        for (int q = 0; q < S; ++q)
+            if (act[q]) build_features(s->sc[q], &feat[(size_t)q * 16]);
+        if (mlp_infer_device(s->mlp, feat.data(), S, s->mstats, s->st)) { set_error("controller inference failed"); return -1; }
+    }
     std::vector<double> eps_h(S);
-    if (read_iteration(s, s->KV, eps_h.data(), true)) return -1;
+    if (read_iteration(s, s->KV, eps_h.data(), true, learned ? mlp_output(s->mlp) : nullptr)) return -1;
     // proposal: the classical law's (kept in c.proposal), replaced by the learned
     // controller's when it is active and its output is usable
     if (features_and_proposal(s, act)) return -1;
@@ -948,9 +1019,9 @@ int features_and_proposal(tsr_solver* s, const std::vector<int>& act)
         return 0;
     }
     if (!(s->ctrl_mode == 1 && s->mlp && mlp_ready(s->mlp))) return 0;
-    std::vector<float> feat((size_t)S * 16, 0.0f), out((size_t)S * 2, 0.0f);
-    for (int q = 0; q < S; ++q) std::memcpy(&feat[(size_t)q * 16], s->sc[q].feat, sizeof(float) * 16);
-    if (mlp_infer(s->mlp, feat.data(), S, out.data(), s->st)) { set_error("controller inference failed"); return -1; }
+    // the network's output for these features was computed on the device and read back
+    // with the step-start results (solver_step)
+    const std::vector<float>& out = s->ctl_h;
     const int hi_cap = std::min(kLearnedCap, s->mcap);
     for (int q = 0; q < S; ++q) {
         if (!act[q]) continue;
@@ -1093,7 +1164,8 @@ tsr_solver* create(const tsr_batch_view* view, const exprem_options_struct* opt)
         c.csW.assign(s->mcap + 2, 0.0);
     }
     // mapped readback buffer: a full window (mcap columns) per scenario, at least 16 doubles
-    s->mb_stride = std::max(16, s->mcap * (s->mcap + 3) / 2 + (s->mcap + 1) + 2 * s->mcap + 1);
+    s->mb_stride = std::max(kPackSmall, s->mcap * (s->mcap + 3) / 2 + (s->mcap + 1) + 2 * s->mcap + 1);
+    s->ctl_h.assign((size_t)S * 2, 0.0f);
     s->mb_stride = (s->mb_stride + 7) & ~7;
     if (hipHostMalloc((void**)&s->mb_h, sizeof(double) * (size_t)S * s->mb_stride,
                       hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
@@ -1182,8 +1254,8 @@ static void release_solver(tsr_solver* s)
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
     if (s->mlp) mlp_free(s->mlp);
+    for (auto& g : s->graphs) hipGraphExecDestroy(g.second.first);
     for (hipEvent_t e : s->kt.all) hipEventDestroy(e);
-    for (auto& g : s->graphs) hipGraphExecDestroy(g.second);
     free_problem(s->P);
     if (s->st) hipStreamDestroy(s->st);
     delete s;
@@ -1326,7 +1398,10 @@ int tsr_solver_ktimer(tsr_solver* s, int every)
     if (hipStreamSynchronize(s->st) == hipSuccess && !s->kt.pend.empty()) kt_harvest(s->kt);
     KTimer& k = s->kt;
     k.every = every;
-    for (int c = 0; c < KT_COUNT; ++c) { k.seen[c] = 0; k.timed[c] = 0; k.ms[c] = 0.0; k.bytes[c] = 0.0; }
+    for (int c = 0; c < KT_COUNT; ++c) { k.seen[c] = 0; k.timed[c] = 0; k.ms[c] = 0.0; k.bytes[c] = 0.0; k.cap_n[c] = 0; }
+    // captured windows carry the timer setting they were captured with: recapture
+    for (auto& g : s->graphs) hipGraphExecDestroy(g.second.first);
+    s->graphs.clear();
     return 0;
 }
 
