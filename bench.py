@@ -304,7 +304,8 @@ def main():
         "accuracy": acc,
         "kernels": kernels,
         "kernel_timing": f"HIP events on the solver stream around every {args.ktimer_every}th launch of each "
-                         "class during the timed steps; bytes are algorithmic (reads and writes the kernel must do)",
+                         "class during the timed steps (with --graph, event-record nodes captured with each Arnoldi "
+                         "window); bytes are algorithmic (reads and writes the kernel must do)",
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": dk["GB_per_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dk["GB_per_s"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_over_algorithmic": traffic_ratio,

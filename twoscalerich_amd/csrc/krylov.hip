@@ -21,6 +21,18 @@
 
 namespace tsr {
 
+// Basis columns arrive as an array of generic pointers.  Reading them through global
+// address-space pointers lets the compiler issue global rather than flat loads: a flat
+// load may alias LDS, so it is ordered against the kernels' LDS stores and waited for
+// one at a time, while global loads of several columns stay in flight together.
+template <class T>
+using gptr = const __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> as_global(const T* p)
+{
+    return (gptr<T>)p;
+}
+
 // wave_sum_v: the DPP wavefront sum of device.h (wave_sum_dpp)
 __device__ __forceinline__ double wave_sum_v(double x) { return wave_sum_dpp(x); }
 
@@ -67,7 +79,7 @@ __global__ __launch_bounds__(kVecBS) void k_pass(int mode, const double* const* 
         const double* cf = coef + (size_t)s * ldc;
         for (int k = 0; k < nk; ++k) {
             const double c = cf[k];
-            const double* col = cols[k];
+            const gptr<double> col = as_global(cols[k]);
 #pragma unroll
             for (int r = 0; r < kVecR; ++r)
                 if (ok[r]) wr[r] -= col[idx[r]] * c;
@@ -94,10 +106,10 @@ __global__ __launch_bounds__(kVecBS) void k_pass(int mode, const double* const* 
     // independent shuffle trees
     int k = 0;
     for (; k + 4 <= nk; k += 4) {
-        const double* c0 = cols[k];
-        const double* c1 = cols[k + 1];
-        const double* c2 = cols[k + 2];
-        const double* c3 = cols[k + 3];
+        const gptr<double> c0 = as_global(cols[k]);
+        const gptr<double> c1 = as_global(cols[k + 1]);
+        const gptr<double> c2 = as_global(cols[k + 2]);
+        const gptr<double> c3 = as_global(cols[k + 3]);
         double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
 #pragma unroll
         for (int r = 0; r < kVecR; ++r)
@@ -115,7 +127,7 @@ __global__ __launch_bounds__(kVecBS) void k_pass(int mode, const double* const* 
         if (lane == 0) { lds[wv][k] = p0; lds[wv][k + 1] = p1; lds[wv][k + 2] = p2; lds[wv][k + 3] = p3; }
     }
     for (; k < nk; ++k) {
-        const double* col = cols[k];
+        const gptr<double> col = as_global(cols[k]);
         double p = 0.0;
 #pragma unroll
         for (int r = 0; r < kVecR; ++r)
@@ -139,6 +151,55 @@ __global__ __launch_bounds__(kVecBS) void k_pass(int mode, const double* const* 
 }
 
 // ---------------------------------------------------------------- DCGS2
+// Four basis columns k..k+3 of a DCGS2 pass, for a thread's kVecR rows: every load of the
+// group is issued before any is used (with FULL, every row of the block exists, so no
+// row test stands between the loads), then the per-row arithmetic in the passes' fixed
+// order.  dca_group: column sums sx[c] += a w, px[c] += a R_j; dcb_group: w -= a coef,
+// R_j -= a coefr, column by column.
+template <bool FULL, class T>
+__device__ __forceinline__ void dca_group(const T* const* cols, int k, const long long* idx, const bool* ok,
+                                          const double* wr, const double* rj, double* sx, double* px)
+{
+    double av[kVecR][4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const gptr<T> col = as_global(cols[k + c]);
+#pragma unroll
+        for (int r = 0; r < kVecR; ++r) av[r][c] = (FULL || ok[r]) ? (double)col[idx[r]] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < kVecR; ++r)
+        if (FULL || ok[r]) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                sx[c] += av[r][c] * wr[r];
+                px[c] += av[r][c] * rj[r];
+            }
+        }
+}
+
+template <bool FULL, class T>
+__device__ __forceinline__ void dcb_group(const T* const* cols, int k, const long long* idx, const bool* ok,
+                                          const double* cf, const double* cr, double* wr, double* rn)
+{
+    double av[kVecR][4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const gptr<T> col = as_global(cols[k + c]);
+#pragma unroll
+        for (int r = 0; r < kVecR; ++r) av[r][c] = (FULL || ok[r]) ? (double)col[idx[r]] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < kVecR; ++r)
+        if (FULL || ok[r]) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                wr[r] -= av[r][c] * cf[k + c];
+                rn[r] -= av[r][c] * cr[k + c];
+            }
+        }
+}
+
 // Classical Gram-Schmidt with the reorthogonalisation of each column delayed by one
 // iteration (the "DCGS2" scheme of Swirydowicz, Langou, Ananthan, Yang and Thomas,
 // Numer. Linear Algebra Appl. 2021), so one Arnoldi iteration reads the basis twice
@@ -174,18 +235,11 @@ __global__ __launch_bounds__(kVecBS) void k_pass_dca(const T* const* cols, const
         rj[r] = ok[r] ? Rj[idx[r]] : 0.0;
     }
     int k = 0;
-    for (; k + 4 <= j; k += 4) {   // four columns at a time: 16 loads in flight per thread
+    const bool full = r1 - r0 == (long long)kVecBS * kVecR;   // every row of the block exists
+    for (; k + 4 <= j; k += 4) {   // four columns at a time
         double sx[4] = {0.0, 0.0, 0.0, 0.0}, px[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int r = 0; r < kVecR; ++r)
-            if (ok[r]) {
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const double a = (double)cols[k + c][idx[r]];
-                    sx[c] += a * wr[r];
-                    px[c] += a * rj[r];
-                }
-            }
+        if (full) dca_group<true>(cols, k, idx, ok, wr, rj, sx, px);
+        else dca_group<false>(cols, k, idx, ok, wr, rj, sx, px);
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             sx[c] = wave_sum_v(sx[c]);
@@ -197,7 +251,7 @@ __global__ __launch_bounds__(kVecBS) void k_pass_dca(const T* const* cols, const
         }
     }
     for (; k < j; ++k) {
-        const T* c0 = cols[k];
+        const gptr<T> c0 = as_global(cols[k]);
         double s0 = 0.0, p0 = 0.0;
 #pragma unroll
         for (int r = 0; r < kVecR; ++r)
@@ -319,29 +373,14 @@ __global__ __launch_bounds__(kVecBS) void k_pass_dcb(const T* const* cols, const
         rn[r] = rj[r];
     }
     int k = 0;
-    for (; k + 4 <= j; k += 4) {   // four columns at a time: 16 loads in flight per thread
-        const T* c0 = cols[k];
-        const T* c1 = cols[k + 1];
-        const T* c2p = cols[k + 2];
-        const T* c3 = cols[k + 3];
-#pragma unroll
-        for (int r = 0; r < kVecR; ++r)
-            if (ok[r]) {
-                const double a0 = (double)c0[idx[r]], a1 = (double)c1[idx[r]], a2 = (double)c2p[idx[r]],
-                             a3 = (double)c3[idx[r]];
-                wr[r] -= a0 * cf[k];
-                rn[r] -= a0 * cr[k];
-                wr[r] -= a1 * cf[k + 1];
-                rn[r] -= a1 * cr[k + 1];
-                wr[r] -= a2 * cf[k + 2];
-                rn[r] -= a2 * cr[k + 2];
-                wr[r] -= a3 * cf[k + 3];
-                rn[r] -= a3 * cr[k + 3];
-            }
+    const bool full = r1 - r0 == (long long)kVecBS * kVecR;   // every row of the block exists
+    for (; k + 4 <= j; k += 4) {   // four columns at a time
+        if (full) dcb_group<true>(cols, k, idx, ok, cf, cr, wr, rn);
+        else dcb_group<false>(cols, k, idx, ok, cf, cr, wr, rn);
     }
     for (; k < j; ++k) {
         const double c = cf[k], c2 = cr[k];
-        const T* col = cols[k];
+        const gptr<T> col = as_global(cols[k]);
 #pragma unroll
         for (int r = 0; r < kVecR; ++r)
             if (ok[r]) {
@@ -419,7 +458,7 @@ __global__ __launch_bounds__(kVecBS) void k_lincomb(double* out, const double* b
     }
     for (int k = 0; k < nk; ++k) {
         const double c = cf[k];
-        const T* col = cols[k];
+        const gptr<T> col = as_global(cols[k]);
 #pragma unroll
         for (int r = 0; r < kVecR; ++r)
             if (ok[r]) acc[r] += (double)col[idx[r]] * c;
@@ -470,7 +509,7 @@ __global__ __launch_bounds__(kVecBS) void k_lincomb2(double* out1, const double*
         a2[r] = 0.0;
     }
     for (int k = 0; k < nk; ++k) {
-        const T* col = cols[k];
+        const gptr<T> col = as_global(cols[k]);
         const bool u1 = k < nk1, u2 = k < nk2;
         const double c1 = u1 ? cf1[k] : 0.0, c2 = u2 ? cf2[k] : 0.0;
 #pragma unroll
