@@ -50,6 +50,46 @@ __device__ __forceinline__ double block_sum(double x, double* red)
     return s;
 }
 
+// Four basis columns k..k+3 of a CGS2 pass for a thread's kVecR rows, every load issued
+// before any is used (with FULL, every row of the block exists): pass_axpy_group
+// w -= a coef column by column, pass_dot_group p[c] += a w.
+template <bool FULL>
+__device__ __forceinline__ void pass_axpy_group(const double* const* cols, int k, const long long* idx, const bool* ok,
+                                                const double* cf, double* wr)
+{
+    double av[kVecR][4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const gptr<double> col = as_global(cols[k + c]);
+#pragma unroll
+        for (int r = 0; r < kVecR; ++r) av[r][c] = (FULL || ok[r]) ? col[idx[r]] : 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int r = 0; r < kVecR; ++r)
+            if (FULL || ok[r]) wr[r] -= av[r][c] * cf[k + c];
+}
+
+template <bool FULL>
+__device__ __forceinline__ void pass_dot_group(const double* const* cols, int k, const long long* idx, const bool* ok,
+                                               const double* wr, double* pp)
+{
+    double av[kVecR][4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const gptr<double> col = as_global(cols[k + c]);
+#pragma unroll
+        for (int r = 0; r < kVecR; ++r) av[r][c] = (FULL || ok[r]) ? col[idx[r]] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < kVecR; ++r)
+        if (FULL || ok[r]) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) pp[c] += av[r][c] * wr[r];
+        }
+}
+
 // part[k][b] below is part[k * ldp + b] (ldp = number of workgroups).
 // mode PASS_DOTS:       part[k][b] = sum_rows col_k * w                     (k < nk), part[nk][b] = sum w^2
 // mode PASS_AXPY_DOTS:  w -= sum_k col_k * coef[s][k]; then as PASS_DOTS (w written back)
@@ -75,9 +115,15 @@ __global__ __launch_bounds__(kVecBS) void k_pass(int mode, const double* const* 
         ok[r] = idx[r] < r1;
         wr[r] = ok[r] ? w[idx[r]] : 0.0;
     }
+    const bool full = r1 - r0 == (long long)kVecBS * kVecR;   // every row of the block exists
     if (mode != PASS_DOTS) {
         const double* cf = coef + (size_t)s * ldc;
-        for (int k = 0; k < nk; ++k) {
+        int k = 0;
+        for (; k + 4 <= nk; k += 4) {   // four columns at a time, the same order
+            if (full) pass_axpy_group<true>(cols, k, idx, ok, cf, wr);
+            else pass_axpy_group<false>(cols, k, idx, ok, cf, wr);
+        }
+        for (; k < nk; ++k) {
             const double c = cf[k];
             const gptr<double> col = as_global(cols[k]);
 #pragma unroll
@@ -102,24 +148,14 @@ __global__ __launch_bounds__(kVecBS) void k_pass(int mode, const double* const* 
         if (tid == 0) { part[b] = s0; part[(size_t)ldp + b] = s1; }
         return;
     }
-    // four columns at a time: 16 independent loads in flight per thread, four
+    // four columns at a time: every load of the group in flight together, four
     // independent shuffle trees
     int k = 0;
     for (; k + 4 <= nk; k += 4) {
-        const gptr<double> c0 = as_global(cols[k]);
-        const gptr<double> c1 = as_global(cols[k + 1]);
-        const gptr<double> c2 = as_global(cols[k + 2]);
-        const gptr<double> c3 = as_global(cols[k + 3]);
-        double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
-#pragma unroll
-        for (int r = 0; r < kVecR; ++r)
-            if (ok[r]) {
-                const long long i = idx[r];
-                p0 += c0[i] * wr[r];
-                p1 += c1[i] * wr[r];
-                p2 += c2[i] * wr[r];
-                p3 += c3[i] * wr[r];
-            }
+        double pp[4] = {0.0, 0.0, 0.0, 0.0};
+        if (full) pass_dot_group<true>(cols, k, idx, ok, wr, pp);
+        else pass_dot_group<false>(cols, k, idx, ok, wr, pp);
+        double p0 = pp[0], p1 = pp[1], p2 = pp[2], p3 = pp[3];
         p0 = wave_sum_v(p0);
         p1 = wave_sum_v(p1);
         p2 = wave_sum_v(p2);
@@ -431,6 +467,20 @@ __global__ __launch_bounds__(kVecBS) void k_norms(const double* a, const double*
     if (threadIdx.x == 0) { part[b] = s0; part[(size_t)ldp + b] = s1; part[(size_t)2 * ldp + b] = s2; }
 }
 
+// Four basis columns k..k+3 at a thread's kVecR rows, every load issued before any is
+// used (with FULL, every row of the block exists; rows past the end read as 0).
+template <bool FULL, class T>
+__device__ __forceinline__ void load_group(const T* const* cols, int k, const long long* idx, const bool* ok,
+                                           double av[kVecR][4])
+{
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const gptr<T> col = as_global(cols[k + c]);
+#pragma unroll
+        for (int r = 0; r < kVecR; ++r) av[r][c] = (FULL || ok[r]) ? (double)col[idx[r]] : 0.0;
+    }
+}
+
 // out = base + sgn * sum_{k < nk_s[s]} col_k * coef[s][k] - extra_sub
 // partmode 0: nothing; 1: part[0][b] = sum out^2; 2: part[0][b] = sum (c/wt(u))^2 of the
 // combination c = sum col_k coef_k alone (the error vector) with out = base + c.
@@ -456,7 +506,19 @@ __global__ __launch_bounds__(kVecBS) void k_lincomb(double* out, const double* b
         ok[r] = idx[r] < r1;
         acc[r] = 0.0;
     }
-    for (int k = 0; k < nk; ++k) {
+    const bool full = r1 - r0 == (long long)kVecBS * kVecR;   // every row of the block exists
+    int k = 0;
+    for (; k + 4 <= nk; k += 4) {   // four columns at a time, the same order
+        double av[kVecR][4];
+        if (full) load_group<true>(cols, k, idx, ok, av);
+        else load_group<false>(cols, k, idx, ok, av);
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int r = 0; r < kVecR; ++r)
+                if (ok[r]) acc[r] += av[r][c] * cf[k + c];
+    }
+    for (; k < nk; ++k) {
         const double c = cf[k];
         const gptr<T> col = as_global(cols[k]);
 #pragma unroll
@@ -508,7 +570,25 @@ __global__ __launch_bounds__(kVecBS) void k_lincomb2(double* out1, const double*
         a1[r] = 0.0;
         a2[r] = 0.0;
     }
-    for (int k = 0; k < nk; ++k) {
+    const bool full = r1 - r0 == (long long)kVecBS * kVecR;   // every row of the block exists
+    int k = 0;
+    for (; k + 4 <= nk; k += 4) {   // four columns at a time, the same order
+        double av[kVecR][4];
+        if (full) load_group<true>(cols, k, idx, ok, av);
+        else load_group<false>(cols, k, idx, ok, av);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const bool u1 = k + c < nk1, u2 = k + c < nk2;
+            const double c1 = u1 ? cf1[k + c] : 0.0, c2 = u2 ? cf2[k + c] : 0.0;
+#pragma unroll
+            for (int r = 0; r < kVecR; ++r)
+                if (ok[r]) {
+                    if (u1) a1[r] += av[r][c] * c1;
+                    if (u2) a2[r] += av[r][c] * c2;
+                }
+        }
+    }
+    for (; k < nk; ++k) {
         const gptr<T> col = as_global(cols[k]);
         const bool u1 = k < nk1, u2 = k < nk2;
         const double c1 = u1 ? cf1[k] : 0.0, c2 = u2 ? cf2[k] : 0.0;
