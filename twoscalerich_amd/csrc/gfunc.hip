@@ -92,7 +92,7 @@ __device__ __forceinline__ void keff_dev(const DScen& S, double h, double K[4])
     const double fr = tt - (double)i;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        const double a = S.keff[c * n + i], b = S.keff[c * n + i + 1];
+        const double a = as_global(S.keff)[c * n + i], b = as_global(S.keff)[c * n + i + 1];
         K[c] = a + fr * (b - a);
     }
 }
@@ -106,7 +106,7 @@ __device__ __forceinline__ double ld_state(const GArgs& a, double ve, long long 
 
 __device__ __forceinline__ double macro_head(const DScen& S, const GArgs& a, double ve, int j)
 {
-    return S.dir[j] ? S.dirichlet_head : ld_state(a, ve, S.u_off + j);
+    return as_global(S.dir)[j] ? S.dirichlet_head : ld_state(a, ve, S.u_off + j);
 }
 
 // Head at state index gi (-1: a Dirichlet macro node, head dh).
@@ -132,10 +132,10 @@ __device__ __forceinline__ void macro_interp(const DScen& S, const GArgs& a, dou
 {
     const int nv = S.nv, ne = S.ne;
     for (int i = 0; i < 4; ++i) { hv[i] = 0.0; al[i] = 0.0; }
-    for (int i = 0; i < nv; ++i) hv[i] = macro_head(S, a, ve, S.el[i * ne + e]);
+    for (int i = 0; i < nv; ++i) hv[i] = macro_head(S, a, ve, as_global(S.el)[i * ne + e]);
     for (int k = 0; k < nv; ++k) {
         double acc = 0.0;
-        for (int i = 0; i < nv; ++i) acc += hv[i] * S.sf[(k * nv + i) * ne + e];
+        for (int i = 0; i < nv; ++i) acc += hv[i] * as_global(S.sf)[(k * nv + i) * ne + e];
         al[k] = acc;
     }
 }
@@ -158,26 +158,26 @@ __device__ __forceinline__ void macro_elem(const DBatch& B, const GArgs& a, int 
     // kernel's register budget with the micro cells)
 #pragma unroll 1
     for (int j = 0; j < nv; ++j) {
-        const double x = S.midx[j * ne + e], y = S.midy[j * ne + e];
+        const double x = as_global(S.midx)[j * ne + e], y = as_global(S.midy)[j * ne + e];
         double gx, gy, hip;
         if (nv == 3) { gx = al[0]; gy = al[1]; hip = al[0] * x + al[1] * y + al[2]; }
         else { gx = al[0] + al[2] * y; gy = al[1] + al[2] * x; hip = al[0] * x + al[1] * y + al[2] * x * y + al[3]; }
         double K[4];
         keff_dev(S, hip, K);
-        const double nx = S.nrm[(2 * j) * ne + e], ny = S.nrm[(2 * j + 1) * ne + e];
-        S.qe[e * nv + j] = -((K[0] * gx + K[1] * (gy + 1.0)) * nx + (K[2] * gx + K[3] * (gy + 1.0)) * ny);
+        const double nx = as_global(S.nrm)[(2 * j) * ne + e], ny = as_global(S.nrm)[(2 * j + 1) * ne + e];
+        as_global_w(S.qe)[e * nv + j] = -((K[0] * gx + K[1] * (gy + 1.0)) * nx + (K[2] * gx + K[3] * (gy + 1.0)) * ny);
     }
 #pragma unroll 1
     for (int p = 0; p < nv; ++p) {
-        const int type = S.bce[p * ne + e];
+        const int type = as_global(S.bce)[p * ne + e];
 #pragma unroll 1
         for (int hk = 0; hk < 2; ++hk) {
             double f = 0.0;
             if (type == TSR_BC_FLUX) {
-                f = S.boundary_flux * S.be_len[(2 * p + hk) * ne + e];
+                f = S.boundary_flux * as_global(S.be_len)[(2 * p + hk) * ne + e];
             } else if (type == TSR_BC_DHDX0 || type == TSR_BC_DHDY0) {
-                const double bx = S.be_mid[(4 * p + 2 * hk) * ne + e], by = S.be_mid[(4 * p + 2 * hk + 1) * ne + e];
-                const double bnx = S.be_nrm[(4 * p + 2 * hk) * ne + e], bny = S.be_nrm[(4 * p + 2 * hk + 1) * ne + e];
+                const double bx = as_global(S.be_mid)[(4 * p + 2 * hk) * ne + e], by = as_global(S.be_mid)[(4 * p + 2 * hk + 1) * ne + e];
+                const double bnx = as_global(S.be_nrm)[(4 * p + 2 * hk) * ne + e], bny = as_global(S.be_nrm)[(4 * p + 2 * hk + 1) * ne + e];
                 double gx, gy, hip;
                 if (nv == 3) { gx = al[0]; gy = al[1]; hip = al[0] * bx + al[1] * by + al[2]; }
                 else { gx = al[0] + al[2] * by; gy = al[1] + al[2] * bx; hip = al[0] * bx + al[1] * by + al[2] * bx * by + al[3]; }
@@ -186,7 +186,7 @@ __device__ __forceinline__ void macro_elem(const DBatch& B, const GArgs& a, int 
                 keff_dev(S, hip, K);
                 f = (K[0] * gx + K[1] * (gy + 1.0)) * bnx + (K[2] * gx + K[3] * (gy + 1.0)) * bny;
             }
-            S.bq[e * 2 * nv + 2 * p + hk] = f;
+            as_global_w(S.bq)[e * 2 * nv + 2 * p + hk] = f;
         }
     }
 }
@@ -204,8 +204,8 @@ __device__ __forceinline__ void macro_bary(const double* m, double px, double py
 __device__ __forceinline__ void dir_point(const DScen& S, const DTmpl& T, double hx, double hy, int i, double& px,
                                           double& py)
 {
-    px = fmin(fmax(hx + T.vec[i], S.box[0]), S.box[1]);
-    py = fmin(fmax(hy + T.vec[T.nn + i], S.box[2]), S.box[3]);
+    px = fmin(fmax(hx + as_global(T.vec)[i], as_global(S.box)[0]), as_global(S.box)[1]);
+    py = fmin(fmax(hy + as_global(T.vec)[T.nn + i], as_global(S.box)[2]), as_global(S.box)[3]);
 }
 
 // Upload-time choice of the macro element that interpolates each micro Dirichlet node
@@ -221,17 +221,17 @@ __global__ __launch_bounds__(256) void k_dir_elem(DBatch B)
     const DScen& S = B.scen[B.cell_scen[cell]];
     const DTmpl& T = B.tmpl[B.cell_tmpl[cell]];
     int best = 0;
-    if (i < T.nn && T.bnd[i] == 1 && S.host == TSR_HOST_NODE && S.model != 1) {
+    if (i < T.nn && as_global(T.bnd)[i] == 1 && S.host == TSR_HOST_NODE && S.model != 1) {
         const int host = cell - S.first_cell;
-        const int q0 = S.ptr[host], ninc = S.ptr[host + 1] - q0;
+        const int q0 = as_global(S.ptr)[host], ninc = as_global(S.ptr)[host + 1] - q0;
         double px, py;
-        dir_point(S, T, S.nodes[host], S.nodes[S.nn + host], i, px, py);
+        dir_point(S, T, as_global(S.nodes)[host], as_global(S.nodes)[S.nn + host], i, px, py);
         double bmin = -INFINITY;
         for (int k = 0; k < ninc; ++k) {
-            const int e = S.inc[q0 + k] >> 2;
+            const int e = as_global(S.inc)[q0 + k] >> 2;
             double m[9], l[3];
 #pragma unroll
-            for (int j = 0; j < 9; ++j) m[j] = S.sf[j * S.ne + e];
+            for (int j = 0; j < 9; ++j) m[j] = as_global(S.sf)[j * S.ne + e];
             macro_bary(m, px, py, l);
             const double mn = fmin(fmin(l[0], l[1]), l[2]);
             if (k == 0 || mn > bmin + 1e-12) { bmin = mn; best = k; }
@@ -257,17 +257,17 @@ __global__ __launch_bounds__(256) void k_cell_macro(DBatch B)
         return;
     }
     const int host = cell - S.first_cell;
-    const int q0 = S.ptr[host], ninc = S.ptr[host + 1] - q0;
+    const int q0 = as_global(S.ptr)[host], ninc = as_global(S.ptr)[host + 1] - q0;
     if (k == 0) {
-        B.cm_host[cell] = S.dir[host] ? -1 : (int)(S.u_off + host);
+        B.cm_host[cell] = as_global(S.dir)[host] ? -1 : (int)(S.u_off + host);
         B.cm_ninc[cell] = (unsigned char)ninc;
     }
     if (k >= ninc) return;
-    const int e = S.inc[q0 + k] >> 2;
-    for (int j = 0; j < 9; ++j) sf[j] = S.sf[j * S.ne + e];
+    const int e = as_global(S.inc)[q0 + k] >> 2;
+    for (int j = 0; j < 9; ++j) sf[j] = as_global(S.sf)[j * S.ne + e];
     for (int v = 0; v < 3; ++v) {
-        const int n = S.el[v * S.ne + e];
-        vi[v] = S.dir[n] ? -1 : (int)(S.u_off + n);
+        const int n = as_global(S.el)[v * S.ne + e];
+        vi[v] = as_global(S.dir)[n] ? -1 : (int)(S.u_off + n);
     }
 }
 
@@ -275,7 +275,7 @@ __global__ __launch_bounds__(256) void k_cell_macro(DBatch B)
 __device__ __forceinline__ int elem_soil(const DBatch& B, const DTmpl& T, int cell, int e)
 {
     if (B.max_discs > 0) {
-        const double cx = T.cen[e], cy = T.cen[T.ne + e];
+        const double cx = as_global(T.cen)[e], cy = as_global(T.cen)[T.ne + e];
         const double* d = B.discs + (size_t)cell * B.max_discs * 3;
         for (int k = 0; k < B.max_discs; ++k) {
             const double r = d[3 * k + 2];
@@ -286,7 +286,7 @@ __device__ __forceinline__ int elem_soil(const DBatch& B, const DTmpl& T, int ce
         }
         return TSR_SOIL_A;
     }
-    return T.soil ? (int)T.soil[e] : TSR_SOIL_B;
+    return T.soil ? (int)as_global(T.soil)[e] : TSR_SOIL_B;
 }
 
 // Upload-time soil map: one thread per (cell, 32-element word).
@@ -323,7 +323,7 @@ __device__ __forceinline__ int dirichlet_node(const DTmpl& T, int b)
         if (b < 3 * rm) return rm * T.r + (3 * rm - b);         // (3rm-b, r-1)
         return (4 * rm - b) * T.r;                              // (0, 4rm-b)
     } else {
-        return T.bnd_list[b];
+        return as_global(T.bnd_list)[b];
     }
 }
 
@@ -340,9 +340,9 @@ __device__ __forceinline__ void for_incident(const DTmpl& T, int i, F&& f)
         if (L && U) f(sLU, 1);
         if (R && U) { f(sRU, 0); f(sRU + 1, 0); }
     } else {
-        const int q1 = T.ptr[i + 1];
-        for (int q = T.ptr[i]; q < q1; ++q) {
-            const int v = T.inc[q];
+        const int q1 = as_global(T.ptr)[i + 1];
+        for (int q = as_global(T.ptr)[i]; q < q1; ++q) {
+            const int v = as_global(T.inc)[q];
             f(v >> 2, v & 3);
         }
     }
@@ -356,7 +356,7 @@ __device__ __forceinline__ bool is_dirichlet(const DTmpl& T, int i)
         const int iy = R > 0 ? (int)((unsigned)i / (unsigned)R) : div_r(T, i), ix = i - iy * r;
         return ix == 0 || iy == 0 || ix == r - 1 || iy == r - 1;
     } else {
-        return T.bnd[i] == 1;
+        return as_global(T.bnd)[i] == 1;
     }
 }
 
@@ -442,12 +442,12 @@ __device__ __forceinline__ double generic_net(const DTmpl& T, int i, const doubl
 {
     const int ne = T.ne;
     double net = 0.0;
-    for (int qq = T.ptr[i]; qq < T.ptr[i + 1]; ++qq) {
-        const int e = T.inc[qq] >> 2, lv = T.inc[qq] & 3;
-        const int v0 = T.el[e], v1 = T.el[ne + e], v2 = T.el[2 * ne + e];
+    for (int qq = as_global(T.ptr)[i]; qq < as_global(T.ptr)[i + 1]; ++qq) {
+        const int e = as_global(T.inc)[qq] >> 2, lv = as_global(T.inc)[qq] & 3;
+        const int v0 = as_global(T.el)[e], v1 = as_global(T.el)[ne + e], v2 = as_global(T.el)[2 * ne + e];
         const double h0 = h[v0], h1 = h[v1], h2 = h[v2];
-        const double gx = T.sf[e] * h0 + T.sf[ne + e] * h1 + T.sf[2 * ne + e] * h2;
-        const double gy = T.sf[3 * ne + e] * h0 + T.sf[4 * ne + e] * h1 + T.sf[5 * ne + e] * h2;
+        const double gx = as_global(T.sf)[e] * h0 + as_global(T.sf)[ne + e] * h1 + as_global(T.sf)[2 * ne + e] * h2;
+        const double gy = as_global(T.sf)[3 * ne + e] * h0 + as_global(T.sf)[4 * ne + e] * h1 + as_global(T.sf)[5 * ne + e] * h2;
         const double* Ks = soil_is_a(sw, e) ? KA : KB;
         const double k0 = Ks[v0], k1 = Ks[v1], k2 = Ks[v2];
         const int jin = lv == 0 ? 2 : lv - 1;   // (lv + 2) % 3: the sub-face entering v_lv
@@ -455,8 +455,8 @@ __device__ __forceinline__ double generic_net(const DTmpl& T, int i, const doubl
 #pragma unroll
         for (int t2 = 0; t2 < 2; ++t2) {
             const int j = t2 == 0 ? jin : lv;
-            const double Kf = T.w[(j * 3) * ne + e] * k0 + T.w[(j * 3 + 1) * ne + e] * k1 + T.w[(j * 3 + 2) * ne + e] * k2;
-            qj[t2] = -Kf * (gx * T.nrm[(2 * j) * ne + e] + (gy + 1.0) * T.nrm[(2 * j + 1) * ne + e]);
+            const double Kf = as_global(T.w)[(j * 3) * ne + e] * k0 + as_global(T.w)[(j * 3 + 1) * ne + e] * k1 + as_global(T.w)[(j * 3 + 2) * ne + e] * k2;
+            qj[t2] = -Kf * (gx * as_global(T.nrm)[(2 * j) * ne + e] + (gy + 1.0) * as_global(T.nrm)[(2 * j + 1) * ne + e]);
         }
         net += qj[0];
         net -= qj[1];
@@ -622,7 +622,7 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
         const long long gh = S.u_off + host;
         const double hu = ld_state(a, ve, gh);
         const double pu = (MODE == 2 && a.v) ? ve * a.v[gh] : 0.0;
-        const bool hdir = S.dir[host] != 0;
+        const bool hdir = as_global(S.dir)[host] != 0;
         hb0 = hdir ? S.dirichlet_head : hu;
         hb1 = hdir ? 0.0 : pu;
     }
@@ -646,17 +646,17 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
             macro_interp(S, a, ve, host, hv, al);
             if (S.nv == 3) { s_bc[0] = al[0]; s_bc[1] = al[1]; s_bc[2] = 0.0; s_bc[3] = al[2]; }
             else { s_bc[0] = al[0]; s_bc[1] = al[1]; s_bc[2] = al[2]; s_bc[3] = al[3]; }
-            s_bc[4] = S.cen[host];
-            s_bc[5] = S.cen[S.ne + host];
+            s_bc[4] = as_global(S.cen)[host];
+            s_bc[5] = as_global(S.cen)[S.ne + host];
         } else {
             s_bc[2] = hb1;   // host perturbation (MODE 2, model 1)
             s_bc[3] = hb0;
-            s_bc[4] = S.nodes[host];
-            s_bc[5] = S.nodes[S.nn + host];
+            s_bc[4] = as_global(S.nodes)[host];
+            s_bc[5] = as_global(S.nodes)[S.nn + host];
         }
     }
     if constexpr (STRUCT) {
-        if (tid < 44) s_geo[tid / 22][tid % 22] = T.sg[tid / 22][tid % 22];
+        if (tid < 44) s_geo[tid / 22][tid % 22] = as_global(T.sg)[tid / 22][tid % 22];
     }
     // MODE 2: interior heads, storage capacities (registers) and conductivities (LDS) of
     // this thread's first two nodes from the recorded expansion; MODE 0/1: their heads
@@ -726,8 +726,8 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
                 d = psum / ws;
             }
         } else {
-            const double X = (model == 1) ? s_bc[4] : s_bc[4] + T.vec[i];
-            const double Y = (model == 1) ? s_bc[5] : s_bc[5] + T.vec[nn + i];
+            const double X = (model == 1) ? s_bc[4] : s_bc[4] + as_global(T.vec)[i];
+            const double Y = (model == 1) ? s_bc[5] : s_bc[5] + as_global(T.vec)[nn + i];
             hi = s_bc[0] * X + s_bc[1] * Y + s_bc[2] * X * Y + s_bc[3];
         }
         h[i] = hi;
@@ -746,7 +746,7 @@ __global__ __launch_bounds__(kMicroMaxBS) __attribute__((amdgpu_waves_per_eu(WPE
             {
                 double cva = 0.0, cvb = 0.0;
                 for_incident<STRUCT>(T, i, [&](int e, int) {
-                    const double sc = STRUCT ? s_geo[e & 1][21] : T.scv[e];
+                    const double sc = STRUCT ? s_geo[e & 1][21] : as_global(T.scv)[e];
                     if (soil_is_a(sw, e)) cva += sc; else cvb += sc;
                 });
                 double cA = 0.0, cB = 0.0, kA = 0.0, kB = 0.0, dcA = 0.0, dcB = 0.0, dkA = 0.0, dkB = 0.0;
@@ -996,8 +996,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE))) v
         if (p >= nper) continue;
         const int i = perim_node(p, r);
         be[q] = B.dir_elem[(size_t)cell * B.nn_max + i];
-        bx[q] = T.vec[i];
-        by[q] = T.vec[nn + i];
+        bx[q] = as_global(T.vec)[i];
+        by[q] = as_global(T.vec)[nn + i];
         if (!node_host) bh[q] = L[i];
     }
     // last, what waits on a loaded index: the macro vertex heads of a node-hosted cell,
@@ -1015,7 +1015,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE))) v
         const long long gh = S.u_off + host;
         hu0 = a.u[gh];
         if (a.v) hv0 = a.v[gh];
-        hdir = S.dir[host] != 0;
+        hdir = as_global(S.dir)[host] != 0;
     }
     const double ve = a.v ? a.veps[s] : 0.0;
     // the same expressions as state_or / ld_state: u + ve v, or the Dirichlet head
@@ -1048,13 +1048,13 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE))) v
             macro_interp(S, a, ve, host, hv, al);
             if (S.nv == 3) { s_bc[0] = al[0]; s_bc[1] = al[1]; s_bc[2] = 0.0; s_bc[3] = al[2]; }
             else { s_bc[0] = al[0]; s_bc[1] = al[1]; s_bc[2] = al[2]; s_bc[3] = al[3]; }
-            s_bc[4] = S.cen[host];
-            s_bc[5] = S.cen[S.ne + host];
+            s_bc[4] = as_global(S.cen)[host];
+            s_bc[5] = as_global(S.cen)[S.ne + host];
         } else {
             s_bc[2] = hb1;
             s_bc[3] = hb0;
-            s_bc[4] = S.nodes[host];
-            s_bc[5] = S.nodes[S.nn + host];
+            s_bc[4] = as_global(S.nodes)[host];
+            s_bc[5] = as_global(S.nodes)[S.nn + host];
         }
     }
     // (a Dirichlet node parks its recorded conductivities and derivatives in its own
@@ -1093,8 +1093,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE))) v
                 d = s_bc[2];
             } else {
                 // dir_point's expressions with the prefetched template position
-                const double px = fmin(fmax(s_bc[4] + bx[q], S.box[0]), S.box[1]);
-                const double py = fmin(fmax(s_bc[5] + by[q], S.box[2]), S.box[3]);
+                const double px = fmin(fmax(s_bc[4] + bx[q], as_global(S.box)[0]), as_global(S.box)[1]);
+                const double py = fmin(fmax(s_bc[5] + by[q], as_global(S.box)[2]), as_global(S.box)[3]);
                 double lam[3];
                 macro_bary(s_mel[be[q]], px, py, lam);
                 double ws = 0.0, hsum = 0.0, psum = 0.0;
@@ -1158,17 +1158,17 @@ __global__ __launch_bounds__(256) void k_macro_node(DBatch B, GArgs a)
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= S.nn) return;
     const int nv = S.nv, ne = S.ne;
-    const int q0 = S.ptr[j], q1 = S.ptr[j + 1];
+    const int q0 = as_global(S.ptr)[j], q1 = as_global(S.ptr)[j + 1];
     // what does not wait on the incidence list is loaded first: the node's state and
     // Dirichlet flag, and (node-hosted model 3) its cell's exchange term
-    const bool dj = S.dir[j] != 0;
+    const bool dj = as_global(S.dir)[j] != 0;
     const double hj = ld_state(a, a.v ? a.veps[s] : 0.0, S.u_off + j);
     const bool qnode = S.model == 3 && S.host != TSR_HOST_ELEMENT;
     double qc = 0.0, qw = 0.0;
     if (qnode) {
         const int cell = S.first_cell + j;
         qc = B.Q[cell];
-        qw = S.cv[j] / B.tmpl[B.cell_tmpl[cell]].cell_area;
+        qw = as_global(S.cv)[j] / B.tmpl[B.cell_tmpl[cell]].cell_area;
     }
     // the incident (element, vertex) pairs in CSR order, kMacroNodeBatch at a time: every
     // pair's index, then every flux of the batch, is loaded before any is summed (two
@@ -1178,7 +1178,7 @@ __global__ __launch_bounds__(256) void k_macro_node(DBatch B, GArgs a)
     for (int qb = q0; qb < q1; qb += kMacroNodeBatch) {
         int ev[kMacroNodeBatch];
 #pragma unroll
-        for (int q = 0; q < kMacroNodeBatch; ++q) ev[q] = qb + q < q1 ? S.inc[qb + q] : 0;
+        for (int q = 0; q < kMacroNodeBatch; ++q) ev[q] = qb + q < q1 ? as_global(S.inc)[qb + q] : 0;
         double f[kMacroNodeBatch][4];
 #pragma unroll
         for (int q = 0; q < kMacroNodeBatch; ++q) {
@@ -1186,10 +1186,10 @@ __global__ __launch_bounds__(256) void k_macro_node(DBatch B, GArgs a)
             if (qb + q >= q1) continue;
             const int e = ev[q] >> 2, lv = ev[q] & 3;
             const int prev = (lv + nv - 1) % nv;
-            f[q][0] = S.qe[e * nv + prev];
-            f[q][1] = S.qe[e * nv + lv];
-            f[q][2] = S.bq[e * 2 * nv + 2 * lv];
-            f[q][3] = S.bq[e * 2 * nv + 2 * prev + 1];
+            f[q][0] = as_global(S.qe)[e * nv + prev];
+            f[q][1] = as_global(S.qe)[e * nv + lv];
+            f[q][2] = as_global(S.bq)[e * 2 * nv + 2 * lv];
+            f[q][3] = as_global(S.bq)[e * 2 * nv + 2 * prev + 1];
         }
 #pragma unroll
         for (int q = 0; q < kMacroNodeBatch; ++q) {
@@ -1204,16 +1204,16 @@ __global__ __launch_bounds__(256) void k_macro_node(DBatch B, GArgs a)
         net -= qc * qw;
     } else if (S.model == 3) {
         for (int qq = q0; qq < q1; ++qq) {
-            const int e = S.inc[qq] >> 2, lv = S.inc[qq] & 3;
+            const int e = as_global(S.inc)[qq] >> 2, lv = as_global(S.inc)[qq] & 3;
             const int cell = S.first_cell + e;
-            net -= B.Q[cell] * S.scv[lv * ne + e] / B.tmpl[B.cell_tmpl[cell]].cell_area;
+            net -= B.Q[cell] * as_global(S.scv)[lv * ne + e] / B.tmpl[B.cell_tmpl[cell]].cell_area;
         }
     }
     double g = 0.0;
     if (!dj) {
         double C, K;
         vg_ck(S.A, hj, B.Ss, C, K);
-        g = net / (S.epsA * C * S.cv[j]);
+        g = net / (S.epsA * C * as_global(S.cv)[j]);
     }
     store_out(B, a, s, S.u_off + j, g);
 }

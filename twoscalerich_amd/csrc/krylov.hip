@@ -21,18 +21,6 @@
 
 namespace tsr {
 
-// Basis columns arrive as an array of generic pointers.  Reading them through global
-// address-space pointers lets the compiler issue global rather than flat loads: a flat
-// load may alias LDS, so it is ordered against the kernels' LDS stores and waited for
-// one at a time, while global loads of several columns stay in flight together.
-template <class T>
-using gptr = const __attribute__((address_space(1))) T*;
-template <class T>
-__device__ __forceinline__ gptr<T> as_global(const T* p)
-{
-    return (gptr<T>)p;
-}
-
 // wave_sum_v: the DPP wavefront sum of device.h (wave_sum_dpp)
 __device__ __forceinline__ double wave_sum_v(double x) { return wave_sum_dpp(x); }
 

@@ -4,6 +4,26 @@
 
 namespace tsr {
 
+// Device arrays reached through pointers held in memory (basis column tables, the problem
+// description's structs) are generic pointers to the compiler.  Reading them through
+// global address-space pointers lets it issue global rather than flat loads: a flat
+// load may alias LDS, so it is ordered against the kernels' LDS accesses and waited for
+// one at a time, while global loads stay in flight together.
+template <class T>
+using gptr = const __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> as_global(const T* p)
+{
+    return (gptr<T>)p;
+}
+template <class T>
+using gptr_w = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr_w<T> as_global_w(T* p)
+{
+    return (gptr_w<T>)p;
+}
+
 // x of lanes the DPP control (row/bank masks, out-of-row sources) leaves out read as 0
 template <int CTRL, int ROW_MASK, int BANK_MASK>
 __device__ __forceinline__ double dpp_or_zero(double x)
