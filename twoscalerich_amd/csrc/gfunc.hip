@@ -519,7 +519,7 @@ __device__ __forceinline__ void store_fd(const DBatch& B, const GArgs& a, int s,
 {
     g = (g - g0v) * a.ieps[s];
     if (!isfinite(g)) atomicOr(B.bad + s, 1);
-    a.out[i] = g;
+    __builtin_nontemporal_store(g, &a.out[i]);
 }
 
 // Launch shape (tuned per problem by tune_micro): workgroup size bs (64..1024, a
