@@ -139,8 +139,9 @@ def main():
     ap.add_argument("--rtol", type=float, default=0.0,
                     help="run both paths at rtol = atol = this value instead of the workload's 1e-6 (matched-accuracy "
                          "comparisons)")
-    ap.add_argument("--graph", action="store_true",
-                    help="batched path with its Arnoldi windows captured into HIP graphs (TSR_PATH_GRAPH)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batched path without HIP graphs (by default each Arnoldi window shape is captured into a "
+                         "HIP graph once and replayed: TSR_PATH_GRAPH)")
     ap.add_argument("--basis", choices=("f64", "f32"), default="f64",
                     help="storage of the Krylov basis columns on the batched path (arithmetic is FP64 either way)")
     ap.add_argument("--kernels-only", action="store_true",
@@ -170,7 +171,7 @@ def main():
         opt.RelTol = opt.AbsTol = args.rtol
     tend = args.tend if args.tend > 0 else p.tend()
     u0 = p.initial()
-    path_fast = F.PATH_BATCHED | (F.PATH_F32BASIS if args.basis == "f32" else 0) | (F.PATH_GRAPH if args.graph else 0)
+    path_fast = F.PATH_BATCHED | (F.PATH_F32BASIS if args.basis == "f32" else 0) | (0 if args.no_graph else F.PATH_GRAPH)
 
     fast = T.Solver(p, opt)
     fast.set_path(path_fast)
@@ -294,7 +295,7 @@ def main():
                    "max_krylov_plain": opt.MaxKrylov, "controller": "learned (Krylov range [5, 60])",
                    "path": "batched (DCGS2" + (" on FP32 copies of the basis" if args.basis == "f32" else
                                                ", FP64 basis") + ", deferred Krylov checks, linearised FD products" +
-                           (", Arnoldi windows replayed from HIP graphs" if args.graph else "") + ")",
+                           ("" if args.no_graph else ", Arnoldi windows replayed from HIP graphs") + ")",
                    "steps_requested": args.steps, "budget_s": args.budget_s},
         "walls_s": [round(x, 3) for x in walls],
         "exprem": {"accepted_steps": st["steps"], "rejected": st["rejects"], "rhs_evals": st["fevals"],
@@ -304,8 +305,9 @@ def main():
         "accuracy": acc,
         "kernels": kernels,
         "kernel_timing": f"HIP events on the solver stream around every {args.ktimer_every}th launch of each "
-                         "class during the timed steps (with --graph, event-record nodes captured with each Arnoldi "
-                         "window); bytes are algorithmic (reads and writes the kernel must do)",
+                         "class during the timed steps (inside the replayed HIP graphs: event-record nodes of a second "
+                         "capture of each window, replayed every 8th time); bytes are algorithmic (reads and writes "
+                         "the kernel must do)",
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": dk["GB_per_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dk["GB_per_s"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_over_algorithmic": traffic_ratio,

@@ -74,6 +74,7 @@ struct KTimer {
     double open_bytes = 0.0;
     hipEvent_t open_a = nullptr;
     bool capturing = false;
+    bool cap_timed = false;                // this capture carries timer pairs
     std::vector<Pend> cap;                 // pairs of the window being captured
     long long cap_seen[KT_COUNT] = {};     // its launches per class
     long long cap_n[KT_COUNT] = {};        // sampling count across captures
@@ -82,6 +83,14 @@ struct KTimer {
 struct GraphTiming {
     std::vector<KTimer::Pend> pairs;
     long long seen[KT_COUNT] = {};
+};
+// One Arnoldi window shape: its graph, and (while the kernel timer is on) a second
+// capture that carries the timer's event pairs, replayed every kGraphTimedEvery-th time
+// so the timed launches are an even sample of the replays.
+struct GraphEntry {
+    hipGraphExec_t exec = nullptr, exec_t = nullptr;
+    GraphTiming gt, gt_t;
+    long long replays = 0;
 };
 
 namespace tsr {
@@ -115,7 +124,7 @@ bool kt_begin(int cls, double bytes, hipStream_t st)
     if (!k || k->every <= 0 || cls < 0 || cls >= KT_COUNT || k->open >= 0) return false;
     if (k->capturing) {
         k->cap_seen[cls]++;
-        if (k->cap_n[cls]++ % k->every) return false;
+        if (!k->cap_timed || k->cap_n[cls]++ % k->every) return false;
         hipEvent_t a = kt_graph_event(*k);
         if (!a || hipEventRecordWithFlags(a, st, hipEventRecordExternal) != hipSuccess) return false;
         k->open = cls; k->open_a = a; k->open_bytes = bytes;
@@ -249,7 +258,7 @@ struct tsr_solver {
     KTimer kt;                // sampled per-kernel timing (off unless tsr_solver_ktimer)
     // TSR_PATH_GRAPH: instantiated graphs of Arnoldi windows, keyed by what their kernel
     // arguments depend on (basis, first and last iteration, path bits, state buffer)
-    std::map<std::tuple<int, int, int, int, const double*>, std::pair<hipGraphExec_t, GraphTiming>> graphs;
+    std::map<std::tuple<int, int, int, int, const double*>, GraphEntry> graphs;
     long long graph_launches = 0, graph_captures = 0;
 };
 
@@ -269,17 +278,20 @@ hipError_t timed_sync(tsr_solver* s)
     return e;
 }
 
-// Stream capture of one Arnoldi window (TSR_PATH_GRAPH): the kernel timer, when on,
-// records its sampled pairs into the graph (KTimer::capturing); a capture left open by an
-// early error return is ended and discarded.
+// Stream capture of one Arnoldi window (TSR_PATH_GRAPH): with timed, the kernel timer
+// records its sampled pairs into the graph (KTimer::capturing); without, it only counts
+// the window's launches.  A capture left open by an early error return is ended and
+// discarded.
+constexpr long long kGraphTimedEvery = 8;
 struct CaptureGuard {
     hipStream_t st;
     bool active = false;
     explicit CaptureGuard(hipStream_t s) : st(s) {}
-    hipError_t begin()
+    hipError_t begin(bool timed)
     {
         if (g_kt) {
             g_kt->capturing = true;
+            g_kt->cap_timed = timed;
             g_kt->cap.clear();
             for (int c = 0; c < KT_COUNT; ++c) g_kt->cap_seen[c] = 0;
         }
@@ -510,12 +522,14 @@ int arnoldi(tsr_solver* s, int which)
         const bool graph = (s->path & TSR_PATH_GRAPH) && !s->host_g && gfunc_tuned(s->P, s->path & TSR_PATH_JVLIN);
         const std::tuple<int, int, int, int, const double*> gkey{which, j0, jt, s->path, s->u};
         hipGraphExec_t gexec = nullptr;
-        GraphTiming* gtime = nullptr;
+        GraphEntry* gent = nullptr;
+        bool gtimed = false;
         CaptureGuard capture(s->st);
         if (graph) {
-            auto it = s->graphs.find(gkey);
-            if (it != s->graphs.end()) { gexec = it->second.first; gtime = &it->second.second; }
-            else HIPCHK(capture.begin());
+            gent = &s->graphs[gkey];
+            gtimed = g_kt && gent->replays++ % kGraphTimedEvery == 0;
+            gexec = gtimed ? gent->exec_t : gent->exec;
+            if (!gexec) HIPCHK(capture.begin(gtimed));
         }
         if (gexec) j = jt;   // replayed below
         for (; j < jt; ++j) {
@@ -545,6 +559,7 @@ int arnoldi(tsr_solver* s, int which)
         // H columns c0..jt-1 changed in this window (DCGS2 also corrects column j0-1)
         const int c0 = dc ? std::max(j0 - 1, 0) : j0;
         if (graph) {
+            GraphTiming& gtime = gtimed ? gent->gt_t : gent->gt;
             if (!gexec) {
                 HIPCHK(launch_pack_window(K, c0, jt, s->bad, s->mask, s->S, s->mb_d, s->mb_stride, s->st));
                 hipGraph_t gr = nullptr;
@@ -553,17 +568,15 @@ int arnoldi(tsr_solver* s, int which)
                 const hipError_t ie = hipGraphInstantiate(&gexec, gr, nullptr, nullptr, 0);
                 hipGraphDestroy(gr);
                 HIPCHK(ie);
-                auto& slot = s->graphs[gkey];
-                slot.first = gexec;
-                slot.second = std::move(gt);
-                gtime = &slot.second;
+                (gtimed ? gent->exec_t : gent->exec) = gexec;
+                gtime = std::move(gt);
                 s->graph_captures++;
             }
             HIPCHK(hipGraphLaunch(gexec, s->st));
             s->graph_launches++;
-            if (g_kt && gtime) {   // this replay's timer pairs (complete at the sync below)
-                for (const auto& p : gtime->pairs) g_kt->pend.push_back(p);
-                for (int c = 0; c < KT_COUNT; ++c) g_kt->seen[c] += gtime->seen[c];
+            if (g_kt) {   // this replay's timer pairs (complete at the sync below) and launches
+                for (const auto& p : gtime.pairs) g_kt->pend.push_back(p);
+                for (int c = 0; c < KT_COUNT; ++c) g_kt->seen[c] += gtime.seen[c];
             }
             HIPCHK(timed_sync(s));
             for (int q = 0; q < s->S; ++q)
@@ -1254,7 +1267,10 @@ static void release_solver(tsr_solver* s)
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
     if (s->mlp) mlp_free(s->mlp);
-    for (auto& g : s->graphs) hipGraphExecDestroy(g.second.first);
+    for (auto& g : s->graphs) {
+        if (g.second.exec) hipGraphExecDestroy(g.second.exec);
+        if (g.second.exec_t) hipGraphExecDestroy(g.second.exec_t);
+    }
     for (hipEvent_t e : s->kt.all) hipEventDestroy(e);
     free_problem(s->P);
     if (s->st) hipStreamDestroy(s->st);
@@ -1400,7 +1416,10 @@ int tsr_solver_ktimer(tsr_solver* s, int every)
     k.every = every;
     for (int c = 0; c < KT_COUNT; ++c) { k.seen[c] = 0; k.timed[c] = 0; k.ms[c] = 0.0; k.bytes[c] = 0.0; k.cap_n[c] = 0; }
     // captured windows carry the timer setting they were captured with: recapture
-    for (auto& g : s->graphs) hipGraphExecDestroy(g.second.first);
+    for (auto& g : s->graphs) {
+        if (g.second.exec) hipGraphExecDestroy(g.second.exec);
+        if (g.second.exec_t) hipGraphExecDestroy(g.second.exec_t);
+    }
     s->graphs.clear();
     return 0;
 }
