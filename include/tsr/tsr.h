@@ -276,8 +276,10 @@ int   tsr_mlp_train(void* mlp, const float* x, const float* t, int n, int epochs
  *                      buffer; the same kernels with the same arguments, so the results
  *                      are those of the path without it, bit for bit.  Windows of a host
  *                      RHS (exprem_c with a host Gfunc) and before the micro-cell kernel is
- *                      tuned run uncaptured; the kernel timer does not see replayed
- *                      windows.
+ *                      tuned run uncaptured.  With the kernel timer on (tsr_solver_ktimer),
+ *                      each window shape also gets a capture carrying the timer's event
+ *                      pairs, replayed every 8th time; turning the timer on or off drops
+ *                      the captured graphs.
  * Returns 0, 1 on unknown bits (or F32BASIS without DCGS2, GRAPH without DEFERRED), 2 on
  * allocation failure. */
 enum { TSR_PATH_PLAIN = 0, TSR_PATH_DCGS2 = 1, TSR_PATH_DEFERRED = 2, TSR_PATH_JVLIN = 4, TSR_PATH_MONOTONE = 8,
