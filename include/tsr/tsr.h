@@ -210,6 +210,15 @@ enum { TSR_KT_MICRO_FULL = 0,   /* k_micro, G or FD product of the full RHS */
 int tsr_solver_ktimer(tsr_solver* s, int every);
 int tsr_solver_ktimer_read(tsr_solver* s, long long* seen, long long* timed, double* ms, double* bytes);
 
+/* Host threads for the per-scenario dense work of a scenario batch (the phi functions
+ * of the Krylov stopping rule, the step and the error estimate; scenario q's work
+ * touches only scenario q, so results do not depend on the thread count).  The pool has
+ * TSR_HOST_THREADS threads if that is set when it is first used, else the hardware
+ * threads (at most 16).  n = 1: every scenario on the stepping thread; n = 0 or n > 1:
+ * the whole pool; n < 0: query.  Returns the threads a batch now uses.  No counterpart
+ * in the reference (its driver steps one problem). */
+int tsr_host_threads(int n);
+
 /* Step size the next step starts from (the CurrentStep of exprem_stats_struct, used when
  * a state from elsewhere is resumed).  Returns 1 unless tau is positive and finite. */
 int tsr_solver_set_step(tsr_solver* s, double tau);

@@ -77,6 +77,30 @@ def test_scenario_batch_matches_single_scenarios():
     pb.close()
 
 
+def test_scenario_batch_same_on_host_pool_and_one_thread():
+    """A batch's per-scenario host work (the phi evaluations of the stopping rule, the step
+    and the error estimate) spread over the host pool gives the same states, bit for bit,
+    and the same step and RHS counts as on the stepping thread alone, on the batched path
+    with the learned controller and on the plain path."""
+    L = T.lib()
+    S = 8
+    pb = T.Problem.config(F.CFG_SCENARIOS, macro_n=21, micro_r=11, n_scen=S)
+    tend = 5e-4
+    try:
+        for path, ctl in ((F.PATH_BATCHED, "learned"), (F.PATH_PLAIN, "classical")):
+            nthr = L.tsr_host_threads(0)
+            ua, sa = _run(pb, path, tend, controller=ctl)
+            assert L.tsr_host_threads(1) == 1
+            ub, sb = _run(pb, path, tend, controller=ctl)
+            print(f"[batched] {S} scenarios, path {path}: {nthr} host threads {sa.steps} steps / {sa.fevals} RHS / "
+                  f"{sa.wall_s:.2f} s, one thread {sb.steps} / {sb.fevals} / {sb.wall_s:.2f} s", flush=True)
+            assert (sa.steps, sa.fevals, sa.rejects) == (sb.steps, sb.fevals, sb.rejects)
+            assert np.array_equal(ua, ub)
+    finally:
+        L.tsr_host_threads(0)
+        pb.close()
+
+
 def test_scenario_batch_on_fp32_basis():
     """The FP32 basis copies with scenarios that stop their Arnoldi windows at different
     dimensions (each keeps the copy of the column it stopped at): every scenario of the

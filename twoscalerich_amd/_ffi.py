@@ -142,6 +142,7 @@ def declare(lib):
         "tsr_solver_time_copy": (C.c_int, [P, C.c_int, _dp]),
         "tsr_solver_micro_shape": (None, [P, _ip, _ip]),
         "tsr_solver_ktimer": (C.c_int, [P, C.c_int]),
+        "tsr_host_threads": (C.c_int, [C.c_int]),
         "tsr_solver_ktimer_read": (C.c_int, [P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _dp, _dp]),
         "tsr_solver_set_step": (C.c_int, [P, C.c_double]),
         "tsr_solver_graph_stats": (None, [P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),

@@ -38,7 +38,7 @@ def build(verbose=True):
         _run([hipcc, "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c", os.path.join(CSRC, src), "-o", obj] + common)
         objs.append(obj)
     tmp = OUT + ".tmp"
-    _run([hipcc, "--offload-arch=" + ARCH, "-shared", "-o", tmp] + objs + ["-lamdhip64"])
+    _run([hipcc, "--offload-arch=" + ARCH, "-shared", "-o", tmp] + objs + ["-lamdhip64", "-pthread"])
     os.replace(tmp, OUT)
     return OUT
 

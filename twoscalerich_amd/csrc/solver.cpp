@@ -16,6 +16,7 @@
 // Arnoldi iteration, update and error estimate is one launch over the whole batch
 // with a per-scenario activity mask, so converged or accepted scenarios cost nothing.
 #include "device.h"
+#include "hostpool.h"
 #include "tsr_internal.h"
 
 #include <algorithm>
@@ -25,6 +26,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <tuple>
@@ -177,6 +179,11 @@ struct ScState {
     std::vector<double> H, HD, phi, y, z, cs, csW;
     double phi_tau = -1.0;    // (tau, m) of the phi_1 matrix currently in phi
     int phi_m = -1;
+    // phi_k(tau HD) e1, k = 1..3, of the error basis at (ph3_tau, ph3_m) (the stopping
+    // rule's last evaluation, reused by the error estimate)
+    std::vector<double> ph3;
+    double ph3_tau = -1.0;
+    int ph3_m = -1;
     exprem_stats_struct stats{};
     // recent history and state summary for the step controller
     double hist_tau[4] = {0, 0, 0, 0};
@@ -328,7 +335,34 @@ struct KtScope {
     ~KtScope() { g_kt = keep; }
 };
 double dense_dummy = 0.0;
-double& dense_acc() { return g_timed ? g_timed->t_dense : dense_dummy; }
+// a host-pool thread evaluating one scenario's dense functions times them here instead
+// (the pool's wall time is what the stepping thread waits for, and is counted once)
+thread_local double* t_dense_tls = nullptr;
+double& dense_acc() { return t_dense_tls ? *t_dense_tls : (g_timed ? g_timed->t_dense : dense_dummy); }
+
+// f(q) for every scenario q with want[q] != 0.  With several such scenarios and more
+// than one host thread they are spread over the host pool (f(q) must touch only scenario
+// q's state and its own entries of shared per-scenario arrays, call no HIP function and
+// leave errors to the caller), and the pool's wall time counts as dense time.  Otherwise
+// they run in order on this thread.
+void for_scenarios(const std::vector<int>& want, const std::function<void(int)>& f)
+{
+    std::vector<int> qs;
+    for (int q = 0; q < (int)want.size(); ++q)
+        if (want[q]) qs.push_back(q);
+    if (qs.size() < 2 || HostPool::get().threads() == 1) {
+        for (const int q : qs) f(q);
+        return;
+    }
+    Tic tic(dense_acc());
+    HostPool::get().run((int)qs.size(), [&](int i) {
+        double own = 0.0;
+        double* keep = t_dense_tls;
+        t_dense_tls = &own;
+        f(qs[i]);
+        t_dense_tls = keep;
+    });
+}
 }
 
 namespace {
@@ -438,14 +472,17 @@ int read_iteration(tsr_solver* s, KrylovState& K, double* eps_out = nullptr, boo
     return 0;
 }
 
+// The estimate at (m, tau); phi_1(tau H_m) is evaluated unless c.phi already holds it
+// (the caches are dropped whenever H changes).
 double krylov_est_main(ScState& c, int m, double tau, int ldh, int ldphi)
 {
-    double dummy = 0.0;
-    Tic tic(g_timed ? g_timed->t_dense : dummy);
-    c.phi_m = -1;
-    if (phi1_pade(c.phi.data(), c.H.data(), tau, m, ldh, ldphi)) return INFINITY;
-    c.phi_tau = tau;
-    c.phi_m = m;
+    if (!(c.phi_m == m && c.phi_tau == tau)) {
+        Tic tic(dense_acc());
+        c.phi_m = -1;
+        if (phi1_pade(c.phi.data(), c.H.data(), tau, m, ldh, ldphi)) return INFINITY;
+        c.phi_tau = tau;
+        c.phi_m = m;
+    }
     return tau * (tau * c.beta * c.hn * std::fabs(c.phi[m - 1])) * c.wnext;
 }
 
@@ -498,7 +535,7 @@ int arnoldi(tsr_solver* s, int which)
     std::vector<int> cap(S, 0), next(S, 0);
     for (int q = 0; q < S; ++q) {
         ScState& c = s->sc[q];
-        if (which == 0) { c.m = 0; c.bd = false; c.conv = false; c.phi_m = -1; } else { c.mD = 0; c.bdD = false; }
+        if (which == 0) { c.m = 0; c.bd = false; c.conv = false; c.phi_m = -1; } else { c.mD = 0; c.bdD = false; c.ph3_m = -1; }
         if (!s->mask_h[q]) continue;
         cap[q] = std::min(which == 0 ? c.m_cap : s->opt.MaxKrylov, s->mcap);
         const int pred = which == 0 ? c.pred_m : c.pred_mD;
@@ -584,13 +621,15 @@ int arnoldi(tsr_solver* s, int which)
         } else if (read_window(s, K, c0, jt)) {
             return 2;
         }
-        for (int q = 0; q < S; ++q) {
-            if (!s->mask_h[q]) continue;
+        // the stopping rule per scenario (its phi evaluations on the host pool for a batch)
+        const std::vector<int> chk = s->mask_h;
+        for_scenarios(chk, [&](int q) {
             ScState& c = s->sc[q];
             c.fe += jt - j0;
-            if (s->bad_h[q]) { c.failed = true; s->mask_h[q] = 0; continue; }
+            if (s->bad_h[q]) { c.failed = true; s->mask_h[q] = 0; return; }
             std::vector<double>& Hq = which == 0 ? c.H : c.HD;
             std::vector<double>& csq = which == 0 ? c.cs : c.csW;
+            if (which == 0) c.phi_m = -1; else c.ph3_m = -1;   // H changes below
             // column col holds rows 0..col+1; the rows below are zero (Hessenberg)
             const double* hw = window_part(s, q, c0, jt, 0);
             for (int col = c0; col < jt; ++col) {
@@ -616,14 +655,15 @@ int arnoldi(tsr_solver* s, int which)
                 c.mD = m; c.hnD = hn; c.wnextD = wn;
                 if (bd) return true;
                 if (m < s->opt.MinKrylov) return false;
-                std::vector<double> ph(3 * (size_t)m);
-                int prc;
-                {
+                if (!(c.ph3_m == m && c.ph3_tau == c.tau)) {
+                    c.ph3.assign(3 * (size_t)m, 0.0);
+                    c.ph3_m = -1;
                     Tic tic(dense_acc());
-                    prc = phik_e1(ph.data(), c.HD.data(), ldh, m, c.tau, 3);
+                    if (phik_e1(c.ph3.data(), c.HD.data(), ldh, m, c.tau, 3)) return false;
+                    c.ph3_m = m;
+                    c.ph3_tau = c.tau;
                 }
-                if (prc) return false;
-                return 2.0 * c.tau * c.betaD * (c.tau * hn * std::fabs(ph[2 * m + m - 1])) * wn <= kKrylovTol;
+                return 2.0 * c.tau * c.betaD * (c.tau * hn * std::fabs(c.ph3[2 * m + m - 1])) * wn <= kKrylovTol;
             };
             auto finish = [&](int m, bool bd) {   // stop at m
                 bool b2;
@@ -663,7 +703,7 @@ int arnoldi(tsr_solver* s, int which)
             }
             if (stop || jt >= cap[q]) s->mask_h[q] = 0;
             else next[q] = jt + 1;
-        }
+        });
     }
     return 0;
 }
@@ -743,16 +783,24 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
     // main Arnoldi over scenarios with beta > 0
     for (int q = 0; q < S; ++q) s->mask_h[q] = act[q] && s->sc[q].beta > 0.0;
     if (arnoldi(s, 0)) return -1;
-    for (int q = 0; q < S; ++q) {
-        if (!act[q]) continue;
-        ScState& c = s->sc[q];
-        if (c.failed) { fail_scen("RHS not finite in the Krylov basis", q); return -1; }
-        if (c.beta > 0.0 && !c.bd) {
-            while (krylov_est_main(c, c.m, c.tau, ldh, s->mcap) > kKrylovTol) {
-                c.tau *= 0.5;
-                c.landing = false;
-                if (c.tau < o.MinStep) { fail_scen("Krylov budget: step below MinStep", q); return -1; }
+    {
+        // the Krylov budget per scenario (host pool for a batch); the first failing
+        // scenario in order reports
+        std::vector<int> why(S, 0);
+        for_scenarios(act, [&](int q) {
+            ScState& c = s->sc[q];
+            if (c.failed) { why[q] = 1; return; }
+            if (c.beta > 0.0 && !c.bd) {
+                while (krylov_est_main(c, c.m, c.tau, ldh, s->mcap) > kKrylovTol) {
+                    c.tau *= 0.5;
+                    c.landing = false;
+                    if (c.tau < o.MinStep) { why[q] = 2; return; }
+                }
             }
+        });
+        for (int q = 0; q < S; ++q) {
+            if (why[q] == 1) { fail_scen("RHS not finite in the Krylov basis", q); return -1; }
+            if (why[q] == 2) { fail_scen("Krylov budget: step below MinStep", q); return -1; }
         }
     }
     // attempts
@@ -761,6 +809,23 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
         int np = 0;
         for (int q = 0; q < S; ++q) np += pend[q];
         if (!np) break;
+        // phi_1(tau H) of every pending scenario whose (m, tau) changed, on the host pool
+        // for a batch (a failed evaluation is repeated below, which reports it)
+        {
+            std::vector<int> need(S, 0);
+            for (int q = 0; q < S; ++q) {
+                const ScState& c = s->sc[q];
+                need[q] = pend[q] && c.beta > 0.0 && !(c.phi_m == c.m && c.phi_tau == c.tau);
+            }
+            for_scenarios(need, [&](int q) {
+                ScState& c = s->sc[q];
+                Tic tic(dense_acc());
+                c.phi_m = -1;
+                if (phi1_pade(c.phi.data(), c.H.data(), c.tau, c.m, ldh, s->mcap)) return;
+                c.phi_tau = c.tau;
+                c.phi_m = c.m;
+            });
+        }
         // u2 = u + V y
         for (int q = 0; q < S; ++q) {
             ScState& c = s->sc[q];
@@ -849,18 +914,30 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
             for (int q = 0; q < S; ++q)
                 if (dq[q] && s->sc[q].failed) { fail_scen("RHS not finite in the error-estimate basis", q); return -1; }
         }
-        // e = 2 tau betaD W phi3(tau HD) e1; unew = u2 + e; err = WRMS(e)
+        // e = 2 tau betaD W phi3(tau HD) e1; unew = u2 + e; err = WRMS(e).  The phi
+        // evaluations first (host pool for a batch), then the coefficients in order.
+        // (the stopping rule's evaluation at (mD, tau) is reused when it is the last one)
+        std::vector<int> want3(S, 0), prc3(S, 0);
+        for (int q = 0; q < S; ++q) {
+            const ScState& c = s->sc[q];
+            want3[q] = pend[q] && dq[q] && !(c.ph3_m == c.mD && c.ph3_tau == c.tau);
+        }
+        for_scenarios(want3, [&](int q) {
+            ScState& c = s->sc[q];
+            c.ph3.assign(3 * (size_t)c.mD, 0.0);
+            c.ph3_m = -1;
+            Tic tic(dense_acc());
+            prc3[q] = phik_e1(c.ph3.data(), c.HD.data(), ldh, c.mD, c.tau, 3);
+            if (!prc3[q]) { c.ph3_m = c.mD; c.ph3_tau = c.tau; }
+        });
         for (int q = 0; q < S; ++q) {
             ScState& c = s->sc[q];
             double* cf = &s->coef_h[(size_t)q * s->KW.ld];
             std::fill(cf, cf + s->KW.ld, 0.0);
             s->nk_h[q] = 0;
             if (!pend[q] || !dq[q]) continue;
-            std::vector<double> ph(3 * (size_t)c.mD);
-            {
-                Tic tic(dense_acc());
-                if (phik_e1(ph.data(), c.HD.data(), ldh, c.mD, c.tau, 3)) { set_error("phi_3 evaluation failed"); return -1; }
-            }
+            if (prc3[q]) { set_error("phi_3 evaluation failed"); return -1; }
+            const std::vector<double>& ph = c.ph3;
             for (int k = 0; k < c.mD; ++k) { c.z[k] = 2.0 * c.tau * c.betaD * ph[2 * c.mD + k]; cf[k] = c.csW[k] * c.z[k]; }
             s->nk_h[q] = c.mD;
         }
@@ -1434,6 +1511,13 @@ int tsr_solver_ktimer_read(tsr_solver* s, long long* seen, long long* timed, dou
         if (bytes) bytes[c] = s->kt.bytes[c];
     }
     return 0;
+}
+
+int tsr_host_threads(int n)
+{
+    if (n > 0) tsr::HostPool::get().set_limit(n);
+    else if (n == 0) tsr::HostPool::get().set_limit(0);
+    return tsr::HostPool::get().threads();
 }
 
 void tsr_solver_graph_stats(const tsr_solver* s, long long* launches, long long* captures)
