@@ -43,6 +43,18 @@ int exprem_c(tsr_gfunc_t Gfunc, int N, double* t, double tend, double* tspan, do
  * src/phipade.cpp:15 (H leading dimension MaxKrylov+1, phiH MaxKrylov).  Returns 0, or 1
  * when the Pade denominator is singular (the reference exits instead, phipade.cpp:162-167). */
 int phipade(double* phiH, double* H, double alpha, int m, int MaxKrylov);
+/* The solver's phi functions (no counterpart in the reference, whose exprem source is
+ * absent).  tsr_phi1_e1: phi_1(alpha*H[0:m,0:m]) e1 (the first column only, into
+ * out[0..m-1]; H leading dimension ldh), which is all the step and its Krylov estimate
+ * need: with flush_denormals = 0 the same values, bit for bit, as the first column of
+ * phipade, at about half its cost for large alpha*H (the doublings form one block of
+ * columns of phi instead of all).  tsr_phik_e1: out[(k-1)*m + i] = [phi_k(tau H) e1]_i,
+ * k = 1..kmax, from the exponential of the augmented matrix (the error estimate's phi_3).
+ * flush_denormals != 0 (the solver's setting): subnormal intermediates are taken as zero
+ * (x86 FTZ/DAZ on the calling thread during the call), which changes only values below
+ * 2.2e-308 and makes stiff evaluations several times faster.  Return 0, or 1 as phipade. */
+int tsr_phi1_e1(double* out, const double* H, double alpha, int m, int ldh, int flush_denormals);
+int tsr_phik_e1(double* out, const double* H, int ldh, int m, double tau, int kmax, int flush_denormals);
 
 /* sqrt(mean((PhiError_i / (AbsTol + RelTol |u_i|))^2)) over host arrays.  include/exprem.h:27 */
 double weighted_norm(double* PhiError, double* u, double AbsTol, double RelTol, int N);

@@ -133,6 +133,56 @@ def test_phipade_oracle_and_product_match_scipy(m, scale, alpha):
     assert rc == 0 and np.max(np.abs(out[:m, :m] - ref)) / np.max(np.abs(ref)) < 1e-11
 
 
+@pytest.mark.parametrize("m", [1, 2, 3, 4, 5, 7, 8, 9, 13, 16, 21, 33, 47, 60, 64, 71, 128])
+def test_solver_phi_functions(m):
+    """The solver's phi functions.  tsr_phi1_e1 (phi_1(tau H) e1, forming only a leading
+    block of columns in the doublings) without flushing gives the first column of
+    phipade's full matrix bit for bit, over scaling exponents 0..~15 (||alpha H|| from 1e-3
+    to 1e4, a stiff decaying spectrum) and a ragged leading dimension; with subnormals
+    flushed (the solver's setting) it stays within 1e-14 of that; it matches scipy's expm
+    of the augmented matrix.  tsr_phik_e1 (phi_1..phi_3 e1) matches scipy and is unchanged
+    by flushing beyond 1e-14."""
+    L = T.lib()
+    rng = np.random.default_rng(m)
+    maxk = 128
+    for scale in (1e-3, 0.3, 4.0, 60.0, 1e3, 1e4):
+        H = np.zeros((maxk + 1, maxk), order="F")
+        H[:m + 1, :m] = _hess(m, m, 1.0) * scale / max(1.0, m ** 0.5)
+        H[:m, :m] += np.diag(-rng.uniform(1.5 * scale, 2.5 * scale, m))   # a decaying (stiff) spectrum
+        full = np.zeros((maxk, maxk), order="F")
+        rc = L.phipade(full.ctypes.data_as(_dp), H.ctypes.data_as(_dp), 0.9, m, maxk)
+        e1 = np.full(m, np.nan)
+        e1f = np.full(m, np.nan)
+        rc1 = L.tsr_phi1_e1(e1.ctypes.data_as(_dp), H.ctypes.data_as(_dp), 0.9, m, maxk + 1, 0)
+        rc2 = L.tsr_phi1_e1(e1f.ctypes.data_as(_dp), H.ctypes.data_as(_dp), 0.9, m, maxk + 1, 1)
+        assert rc == rc1 == rc2 == 0, (m, scale)
+        assert np.all(np.isfinite(full[:m, 0])), (m, scale)
+        assert np.array_equal(e1, full[:m, 0]), (m, scale, np.max(np.abs(e1 - full[:m, 0])))
+        assert np.max(np.abs(e1f - e1)) <= 1e-14 * np.max(np.abs(e1)), (m, scale)
+        if scale <= 60.0:
+            A = 0.9 * H[:m, :m]
+            Aug = np.zeros((m + 1, m + 1))
+            Aug[:m, :m] = A
+            Aug[0, m] = 1.0
+            ref = scipy.linalg.expm(Aug)[:m, m]
+            assert np.max(np.abs(e1 - ref)) <= 1e-10 * max(1.0, np.max(np.abs(ref))), (m, scale)
+        if m <= 64:
+            pk = np.full(3 * m, np.nan)
+            pkf = np.full(3 * m, np.nan)
+            assert L.tsr_phik_e1(pk.ctypes.data_as(_dp), H.ctypes.data_as(_dp), maxk + 1, m, 0.9, 3, 0) == 0
+            assert L.tsr_phik_e1(pkf.ctypes.data_as(_dp), H.ctypes.data_as(_dp), maxk + 1, m, 0.9, 3, 1) == 0
+            assert np.max(np.abs(pkf - pk)) <= 1e-14 * np.max(np.abs(pk)), (m, scale)
+            if scale <= 60.0:
+                Aug = np.zeros((m + 3, m + 3))
+                Aug[:m, :m] = 0.9 * H[:m, :m]
+                Aug[0, m] = 1.0
+                Aug[m, m + 1] = Aug[m + 1, m + 2] = 1.0
+                E = scipy.linalg.expm(Aug)
+                for k in range(3):
+                    ref = E[:m, m + k]
+                    assert np.max(np.abs(pk[k * m:(k + 1) * m] - ref)) <= 1e-10 * max(1.0, np.max(np.abs(ref))), (m, scale, k)
+
+
 def test_phipade_scalar_and_zero():
     maxk = 4
     H = np.zeros((maxk + 1, maxk))

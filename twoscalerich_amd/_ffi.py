@@ -108,6 +108,8 @@ def declare(lib):
         "exprem_c": (C.c_int, [C.c_void_p, C.c_int, _dp, C.c_double, _dp, _dp, _dp, user_data_struct,
                                exprem_options_struct, C.POINTER(exprem_stats_struct)]),
         "phipade": (C.c_int, [_dp, _dp, C.c_double, C.c_int, C.c_int]),
+        "tsr_phi1_e1": (C.c_int, [_dp, _dp, C.c_double, C.c_int, C.c_int, C.c_int]),
+        "tsr_phik_e1": (C.c_int, [_dp, _dp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]),
         "weighted_norm": (C.c_double, [_dp, _dp, C.c_double, C.c_double, C.c_int]),
         "check_step": (None, [C.c_double, C.c_double]),
         "check_flag_Gfunc": (C.c_double, [C.c_int, C.c_double]),

@@ -12,6 +12,8 @@
 // reference's (e^a - 1) a / log(e^a) / a is 0/0 there).
 #include "tsr_internal.h"
 
+#include <xmmintrin.h>
+
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -27,15 +29,21 @@ namespace {
 // C <- alpha * A * B + beta * C, n x n column-major.  Each element of A B is summed over k
 // in increasing order (a zero entry of B adds a zero product, which changes no finite
 // sum); blocks of 8 rows x 4 columns keep their 32 sums in registers, so every loaded
-// element of A serves four columns.
+// element of A serves four columns.  With jc < n only the columns of the leading blocks of
+// four that cover columns 0..jc-1 are formed and updated, each by the same code as in
+// the full product (so bit for bit the same values); the other columns of C are left.
 typedef double v4d __attribute__((vector_size(32)));
 #define TSR_LOAD4(v, p) std::memcpy(&(v), (p), sizeof(v4d))
 #define TSR_STORE4(p, v) std::memcpy((p), &(v), sizeof(v4d))
 
-TSR_HOST_CLONES void mm(int n, double alpha, const double* A, const double* B, double beta, double* C, double* tmp)
+TSR_HOST_CLONES void mm(int n, double alpha, const double* A, const double* B, double beta, double* C, double* tmp,
+                        int jc = -1)
 {
     const int n8 = n & ~7, n4 = n & ~3;
-    for (int j0 = 0; j0 < n4; j0 += 4) {
+    // columns formed: whole blocks of four up to jc, or every column
+    const int nc = (jc < 0 || jc >= n) ? n : std::min(n, (jc + 3) & ~3);
+    const int nb4 = std::min(n4, nc);
+    for (int j0 = 0; j0 < nb4; j0 += 4) {
         const double* b0 = B + (size_t)j0 * n;
         const double* b1 = b0 + n;
         const double* b2 = b1 + n;
@@ -67,7 +75,7 @@ TSR_HOST_CLONES void mm(int n, double alpha, const double* A, const double* B, d
                 tmp[i + (size_t)(j0 + c) * n] = acc;
             }
     }
-    for (int j = n4; j < n; ++j) {
+    for (int j = n4; j < nc; ++j) {
         const double* bj = B + (size_t)j * n;
         for (int i = 0; i < n; ++i) {
             double acc = 0.0;
@@ -75,7 +83,7 @@ TSR_HOST_CLONES void mm(int n, double alpha, const double* A, const double* B, d
             tmp[i + (size_t)j * n] = acc;
         }
     }
-    const size_t nn = (size_t)n * n;
+    const size_t nn = (size_t)n * nc;
     if (beta == 0.0) for (size_t i = 0; i < nn; ++i) C[i] = alpha * tmp[i];
     else for (size_t i = 0; i < nn; ++i) C[i] = alpha * tmp[i] + beta * C[i];
 }
@@ -147,10 +155,34 @@ TSR_HOST_CLONES bool solve(int n, double* A, double* B, int nrhs, double* R)
     return true;
 }
 
+// Flush-to-zero and denormals-are-zero (MXCSR bits 15 and 6) on this thread within the
+// scope.  The squarings of a stiff tau*H (eigenvalues far in the left half-plane) drive
+// entries of e^A into the subnormal range (below 2.2e-308), where every x86 operation on
+// them takes a microcode assist; taken as zero they change only entries of that size,
+// and a stiff phi_1 evaluation runs 3-7x faster.
+struct FlushDenormals {
+    unsigned saved = 0;
+    bool on;
+    explicit FlushDenormals(bool f) : on(f)
+    {
+        if (on) { saved = _mm_getcsr(); _mm_setcsr(saved | 0x8040u); }
+    }
+    ~FlushDenormals()
+    {
+        if (on) _mm_setcsr(saved);
+    }
+};
+
 }  // namespace
 
-int phi1_pade(double* phiH, const double* H, double alpha, int m, int ldh, int ldphi)
+int phi1_pade(double* phiH, const double* H, double alpha, int m, int ldh, int ldphi, int ncol, bool flush)
 {
+    const FlushDenormals ftz(flush);
+    // ncol < m: only the leading columns of phi are wanted.  The Pade stage and the
+    // exponential's squarings need whole matrices; the doublings of phi itself then form
+    // only the leading block of four columns (one product in two of each doubling), with
+    // the same operations as the whole product, so those columns are the same bit for bit.
+    const int jc = (ncol > 0 && ncol < m) ? ncol : m;
     // a non-finite entry leaves the Pade denominator undefined: reported as singular
     // (the scaling exponent below would otherwise convert an infinite logarithm to int)
     for (int j = 0; j < m; ++j)
@@ -183,17 +215,17 @@ int phi1_pade(double* phiH, const double* H, double alpha, int m, int ldh, int l
     if (s > 0) {
         mm(m, 1.0, A, N, 1.0, E, tmp.data());
         std::memcpy(T, N, n2 * sizeof(double));
-        mm(m, 0.5, E, N, 0.5, T, tmp.data());
+        mm(m, 0.5, E, N, 0.5, T, tmp.data(), jc);
         std::memcpy(N, T, n2 * sizeof(double));
         for (int k = 1; k < s; ++k) {
             mm(m, 1.0, E, E, 0.0, T, tmp.data());
             std::memcpy(E, T, n2 * sizeof(double));
             std::memcpy(T, N, n2 * sizeof(double));
-            mm(m, 0.5, E, N, 0.5, T, tmp.data());
+            mm(m, 0.5, E, N, 0.5, T, tmp.data(), jc);
             std::memcpy(N, T, n2 * sizeof(double));
         }
     }
-    for (int j = 0; j < m; ++j)
+    for (int j = 0; j < jc; ++j)
         for (int i = 0; i < m; ++i) phiH[i + (size_t)j * ldphi] = N[i + (size_t)j * m];
     return 0;
 }
@@ -226,8 +258,9 @@ int expm_pade6(double* E, const double* A0, int n)
 }
 
 // out[(k-1)*m + i] = [phi_k(tau H) e1]_i for k = 1..kmax (Saad 1992; Sidje 1998, Thm 1).
-int phik_e1(double* out, const double* H, int ldh, int m, double tau, int kmax)
+int phik_e1(double* out, const double* H, int ldh, int m, double tau, int kmax, bool flush)
 {
+    const FlushDenormals ftz(flush);
     const int n = m + kmax;
     std::vector<double> A((size_t)n * n, 0.0), E((size_t)n * n);
     for (int j = 0; j < m; ++j)
@@ -245,4 +278,16 @@ int phik_e1(double* out, const double* H, int ldh, int m, double tau, int kmax)
 extern "C" int phipade(double* phiH, double* H, double alpha, int m, int MaxKrylov)
 {
     return tsr::phi1_pade(phiH, H, alpha, m, MaxKrylov + 1, MaxKrylov);
+}
+
+extern "C" int tsr_phi1_e1(double* out, const double* H, double alpha, int m, int ldh, int flush_denormals)
+{
+    if (m < 1 || ldh < m) return 1;
+    return tsr::phi1_pade(out, H, alpha, m, ldh, m, 1, flush_denormals != 0);
+}
+
+extern "C" int tsr_phik_e1(double* out, const double* H, int ldh, int m, double tau, int kmax, int flush_denormals)
+{
+    if (m < 1 || ldh < m || kmax < 1) return 1;
+    return tsr::phik_e1(out, H, ldh, m, tau, kmax, flush_denormals != 0);
 }

@@ -177,7 +177,7 @@ struct ScState {
     double beta = 0.0, hn = 0.0, hnD = 0.0, betaD = 0.0, err = 0.0, eps = 0.0, wnext = 0.0, wnextD = 0.0;
     bool bd = false, bdD = false, conv = false, failed = false, accepted = false;
     std::vector<double> H, HD, phi, y, z, cs, csW;
-    double phi_tau = -1.0;    // (tau, m) of the phi_1 matrix currently in phi
+    double phi_tau = -1.0;    // (tau, m) of phi_1(tau H_m) e1, the first column of phi
     int phi_m = -1;
     // phi_k(tau HD) e1, k = 1..3, of the error basis at (ph3_tau, ph3_m) (the stopping
     // rule's last evaluation, reused by the error estimate)
@@ -479,7 +479,7 @@ double krylov_est_main(ScState& c, int m, double tau, int ldh, int ldphi)
     if (!(c.phi_m == m && c.phi_tau == tau)) {
         Tic tic(dense_acc());
         c.phi_m = -1;
-        if (phi1_pade(c.phi.data(), c.H.data(), tau, m, ldh, ldphi)) return INFINITY;
+        if (phi1_pade(c.phi.data(), c.H.data(), tau, m, ldh, ldphi, 1, true)) return INFINITY;
         c.phi_tau = tau;
         c.phi_m = m;
     }
@@ -659,7 +659,7 @@ int arnoldi(tsr_solver* s, int which)
                     c.ph3.assign(3 * (size_t)m, 0.0);
                     c.ph3_m = -1;
                     Tic tic(dense_acc());
-                    if (phik_e1(c.ph3.data(), c.HD.data(), ldh, m, c.tau, 3)) return false;
+                    if (phik_e1(c.ph3.data(), c.HD.data(), ldh, m, c.tau, 3, true)) return false;
                     c.ph3_m = m;
                     c.ph3_tau = c.tau;
                 }
@@ -821,7 +821,7 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
                 ScState& c = s->sc[q];
                 Tic tic(dense_acc());
                 c.phi_m = -1;
-                if (phi1_pade(c.phi.data(), c.H.data(), c.tau, c.m, ldh, s->mcap)) return;
+                if (phi1_pade(c.phi.data(), c.H.data(), c.tau, c.m, ldh, s->mcap, 1, true)) return;
                 c.phi_tau = c.tau;
                 c.phi_m = c.m;
             });
@@ -836,7 +836,7 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
             if (!(c.phi_m == c.m && c.phi_tau == c.tau)) {
                 Tic tic(dense_acc());
                 c.phi_m = -1;
-                if (phi1_pade(c.phi.data(), c.H.data(), c.tau, c.m, ldh, s->mcap)) { set_error("phipade: singular Pade denominator"); return -1; }
+                if (phi1_pade(c.phi.data(), c.H.data(), c.tau, c.m, ldh, s->mcap, 1, true)) { set_error("phipade: singular Pade denominator"); return -1; }
                 c.phi_tau = c.tau;
                 c.phi_m = c.m;
             }
@@ -927,7 +927,7 @@ static int solver_step(tsr_solver* s, double tend, tsr_step_record* rec)
             c.ph3.assign(3 * (size_t)c.mD, 0.0);
             c.ph3_m = -1;
             Tic tic(dense_acc());
-            prc3[q] = phik_e1(c.ph3.data(), c.HD.data(), ldh, c.mD, c.tau, 3);
+            prc3[q] = phik_e1(c.ph3.data(), c.HD.data(), ldh, c.mD, c.tau, 3, true);
             if (!prc3[q]) { c.ph3_m = c.mD; c.ph3_tau = c.tau; }
         });
         for (int q = 0; q < S; ++q) {

@@ -21,9 +21,13 @@ int  read_msh(const char* path, bool micro, double& cw, double& ch, int& nv, std
 int  read_keff(const char* path, Keff_struct& K);
 
 // Small dense routines on the Krylov Hessenberg matrix (dense.cpp).
-int phi1_pade(double* phiH, const double* H, double alpha, int m, int ldh, int ldphi);
+// phi_1(alpha H) of the leading m x m block of H (leading dimension ldh) into phiH
+// (leading dimension ldphi); ncol > 0 writes only its first ncol columns (cheaper); flush
+// evaluates with subnormals flushed to zero (faster for stiff alpha H).  Returns 1 on a
+// singular Pade denominator or a non-finite alpha H.
+int phi1_pade(double* phiH, const double* H, double alpha, int m, int ldh, int ldphi, int ncol = 0, bool flush = false);
 int expm_pade6(double* E, const double* A, int n);
-int phik_e1(double* out, const double* H, int ldh, int m, double tau, int kmax);
+int phik_e1(double* out, const double* H, int ldh, int m, double tau, int kmax, bool flush = false);
 
 void set_error(const std::string& s);
 
