@@ -131,9 +131,10 @@ def test_learned_controller_gain():
     print(f"[batched] plain workload to t={tend}: classical {sc.steps} steps / {sc.fevals} RHS / {sc.wall_s:.1f} s, "
           f"learned {sl.steps} / {sl.fevals} / {sl.wall_s:.1f} s; rel diff vs plain: classical {d_c:.2e}, "
           f"learned {d_l:.2e}", flush=True)
-    # measured on one MI355X: 25,711 vs 27,659 RHS evaluations, 10.2 vs 11.5 s
+    # measured on one MI355X: 25,711 vs 27,659 RHS evaluations, 10.2 vs 11.5 s.  The gain is
+    # asserted in RHS evaluations (deterministic); the wall times are printed only, since
+    # they depend on the box
     assert sl.fevals < 0.97 * sc.fevals
-    assert sl.wall_s < sc.wall_s
     assert sl.rejects <= sc.rejects + 5
     # both differ from the plain path by its own global error (~3e-6 against a
     # tight-tolerance solution at this time, profiles/)
